@@ -1,0 +1,289 @@
+/*
+ * laamd: MI355X (gfx950) sparse hot path behind a C ABI.
+ *
+ * CSR SpMV (row-parallel, segmented-sum and ALBUS nnz-balanced forms),
+ * level-scheduled and sync-free sparse triangular solves, the ILU(k)/IC(k)
+ * preconditioner apply built on them, and the PCG / FGMRES loops that drive
+ * them, with FP64 and mixed FP32/BF16 precision. Host-side symbolic work
+ * (ILU(k)/IC(k) fill, level sets, RCM) is plain C++ in the same library and
+ * needs no GPU.
+ *
+ * Conventions (mirroring the reference's free functions, which take raw
+ * pointers and an explicit base, sparse_mat_op/spmv.hpp:16-19):
+ *  - indices are int32, `base` is 0 or 1 and applies to ai and aj;
+ *  - host-side functions take host pointers; functions on la_mat_t /
+ *    la_trsv_t / la_prec_t handles take DEVICE pointers for vectors;
+ *  - every function returns an la_status; 0 = LA_OK. Nothing throws across
+ *    the ABI. la_last_error() returns the text of the calling thread's
+ *    last error;
+ *  - handles own their device memory; host arrays are only read during the
+ *    call. One handle must not be used from two threads at once (the
+ *    reference's functors have the same rule: their const operator()
+ *    mutates scratch, sparse_mat_op/spmv.hpp:181-184).
+ *
+ * Reference interfaces each entry point replaces are cited file:line.
+ */
+#ifndef LAAMD_H
+#define LAAMD_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  LA_OK = 0,
+  LA_ERR_ARG = 1,          /* bad argument (sizes, null pointers, enum)   */
+  LA_ERR_HIP = 2,          /* HIP runtime error                           */
+  LA_ERR_ALLOC = 3,        /* device or host allocation failed            */
+  LA_ERR_NOT_CONVERGED = 4,/* iterative solve hit its iteration limit     */
+  LA_ERR_NUMERIC = 5,      /* missing diagonal / negative or zero pivot   */
+  LA_ERR_STATE = 6,        /* handle not prepared for this call           */
+  LA_ERR_TIMEOUT = 7       /* a sync-free wait exceeded its spin bound    */
+} la_status;
+
+typedef enum { LA_F64 = 0, LA_F32 = 1, LA_BF16 = 2 } la_dtype;
+
+/* sparse_mat_op/spmv.hpp: SerialSPMV/ParallelSPMV (:12-57) -> ROW,
+ * SegSumSPMV (:59-185) -> SEGSUM, ALBUSSPMV (:186-300) -> ALBUS. AUTO picks
+ * ROW or ALBUS from the row-length profile. */
+typedef enum {
+  LA_SPMV_ROW = 0,
+  LA_SPMV_SEGSUM = 1,
+  LA_SPMV_ALBUS = 2,
+  LA_SPMV_AUTO = 3
+} la_spmv_alg;
+
+/* triangle_solve.hpp:147-208 OptimizedTriangularSolve modes: Barrier ->
+ * LEVEL (one launch per level), NoBarrier -> SYNCFREE (one launch, per-row
+ * readiness). */
+typedef enum { LA_TRSV_LEVEL = 0, LA_TRSV_SYNCFREE = 1 } la_trsv_mode;
+
+typedef enum { LA_LOWER = 0, LA_UPPER = 1 } la_uplo;
+
+const char *la_last_error(void);
+int la_version(void);
+
+/* ====================== host-side algorithms (no GPU) ================== */
+
+/* matrix_utils.hpp:637-675 TopologicalSort2<L|U>. iperm[n] (base-adjusted),
+ * prefix[nlevels+1] (caller gives n+1 ints). Input strictly triangular. */
+int la_topo_sort2(int n, int base, const int *ai, const int *aj, int upper,
+                  int *iperm, int *prefix, int *nlevels);
+/* matrix_utils.hpp:720-803 SplitLDU. Two-phase: pass NULL Laj/Lav/Uaj/Uav
+ * to get Lai, Uai, *lnnz, *unnz; then call again with buffers. */
+int la_split_ldu(int n, int base, const int *ai, const int *aj,
+                 const double *av, int *Lai, int *Laj, double *Lav, double *D,
+                 int *Uai, int *Uaj, double *Uav, int *lnnz, int *unnz);
+/* matrix_utils.hpp:805-879 SplitTriangle<U|L> (keeps the diagonal). */
+int la_split_triangle(int n, int base, const int *ai, const int *aj,
+                      const double *av, int upper, int *Tai, int *Taj,
+                      double *Tav, int *tnnz);
+/* matrix_utils.hpp:677-704 Diagonal. *all_present = 0 if one is missing. */
+int la_diagonal(int n, int base, const int *ai, const int *aj,
+                const double *av, int *diagpos, double *diag,
+                int *all_present);
+/* matrix_utils.hpp:311-390 ParallelTranspose2 (stable). tv may be NULL. */
+int la_transpose(int rows, int cols, int base, const int *ai, const int *aj,
+                 const double *av, int *ti, int *tj, double *tv);
+/* matrix_utils.hpp:429-472 permute: rows by iperm, columns by perm (either
+ * may be NULL). */
+int la_permute(int n, int base, const int *ai, const int *aj,
+               const double *av, const int *iperm, const int *perm, int *pai,
+               int *paj, double *pav);
+/* reordering/Reordering.cpp:37-136 SerialCM (reverse Cuthill-McKee).
+ * iperm[new] = old + base; perm its inverse. */
+int la_rcm(int n, int base, const int *ai, const int *aj, int *iperm,
+           int *perm);
+/* mkl_wrapper/incomplete_lu.cpp:153-275 incomplete_lu_k::symbolic_factorize.
+ * Two-phase (faj NULL: fai and *nnz only). first_upper[n] are 0-based
+ * positions of the first column > i. */
+int la_iluk_symbolic(int n, int base, const int *ai, const int *aj, int level,
+                     int *fai, int *faj, int *first_upper, int64_t *nnz);
+/* mkl_wrapper/incomplete_lu.cpp:277-325 numeric_factorize: combined LU
+ * storage (strict-L multipliers, diagonal, strict U). */
+int la_iluk_numeric(int n, int base, const int *ai, const int *aj,
+                    const double *av, const int *fai, const int *faj,
+                    const int *first_upper, double *fav);
+/* sparse_mat_op/precond.hpp:39-143 ICCLevelSymbolic (upper triangle input
+ * incl. diagonal, diag_pos base-adjusted). Two-phase. */
+int la_icc_symbolic(int n, int base, const int *ai, const int *aj,
+                    const int *diag_pos, int lvl, int *icc_ai, int *icc_aj,
+                    int64_t *nnz);
+/* sparse_mat_op/precond.hpp:615-693 ICCLevelNumeric with multiplicative
+ * diagonal shift omega; *ok = 0 on a negative pivot. */
+int la_icc_numeric(int n, int base, const int *ai, const int *aj,
+                   const double *av, const int *diag_pos, double omega,
+                   const int *icc_ai, const int *icc_aj, double *icc_av,
+                   int *ok);
+
+/* ====================== seeded synthetic inputs (host) ================= */
+/* Two-phase where the nnz is not closed-form: pass aj == NULL to get ai and
+ * *nnz, then call again with aj/av. Values are deterministic for a seed. */
+int la_gen_poisson2d(int nx, int ny, int *ai, int *aj, double *av,
+                     int64_t *nnz);
+int la_gen_laplace3d(int nx, int ny, int nz, int *ai, int *aj, double *av,
+                     int64_t *nnz);
+int la_gen_powerlaw(int n, double mean_len, double sigma, int max_len,
+                    uint64_t seed, int *ai, int *aj, double *av,
+                    int64_t *nnz);
+int la_gen_aniso27(int nx, int ny, int nz, double aniso, uint64_t seed,
+                   int *ai, int *aj, double *av, int64_t *nnz);
+/* 2-D 5-point variable-coefficient diffusion: ne = 2*nx*ny-nx-ny interior
+ * edges; theta[ne]; A_ii = sum of incident theta + 1 per missing neighbour
+ * (Dirichlet), A_ij = -theta_e. edge_i/edge_j[ne] list the edge ends;
+ * val_map[4*ne] gives, per edge, the CSR positions of (i,j),(j,i),(i,i),
+ * (j,j). */
+int la_gen_diffusion2d(int nx, int ny, const double *theta, int *ai, int *aj,
+                       double *av, int *edge_i, int *edge_j, int *val_map,
+                       int64_t *nnz);
+int la_gen_uniform(int64_t n, uint64_t seed, double lo, double hi,
+                   double *out);
+int la_gen_lognormal(int64_t n, uint64_t seed, double mu, double sigma,
+                     double *out);
+
+/* ====================== device CSR matrix ============================== */
+typedef struct la_mat_s *la_mat_t;
+
+/* Upload a CSR matrix (host arrays, any base) for SpMV: the handle behind
+ * SPMV<CSR,Impl>::setMatrix (sparse_mat_op/spmv.hpp:317) and
+ * mkl_sparse_mat (mkl_wrapper/mkl_sparse_mat.h:9-123). Values kept in FP64;
+ * la_mat_add_dtype adds an FP32 or BF16 (round-to-nearest-even) copy. */
+int la_mat_create(int rows, int cols, int base, const int *ai, const int *aj,
+                  const double *av, la_mat_t *out);
+int la_mat_destroy(la_mat_t A);
+int la_mat_add_dtype(la_mat_t A, la_dtype dt);
+/* new values on the same pattern (host or device FP64 array, nnz long);
+ * every stored precision is refreshed */
+int la_mat_set_values(la_mat_t A, const double *av_host);
+int la_mat_set_values_device(la_mat_t A, const double *av_dev);
+int la_mat_set_stream(la_mat_t A, void *hip_stream);
+int la_mat_info(la_mat_t A, int *rows, int *cols, int64_t *nnz);
+/* SegSumSPMV/ALBUSSPMV::preprocess (sparse_mat_op/spmv.hpp:68-77,195-228):
+ * nnz-balanced segment table. Called implicitly on first use. */
+int la_spmv_preprocess(la_mat_t A, la_spmv_alg alg);
+/* y = alpha*op(A)*x + beta*y (device pointers), the functor call
+ * operator()(size, base, ai, aj, av, b, x, alpha, beta) of
+ * sparse_mat_op/spmv.hpp:16-29 with b -> x and x -> y, and
+ * mkl_sparse_mat::mult_vec (mkl_sparse_mat.cpp:325-329). beta == 0 never
+ * reads y (the SerialSPMV rule, spmv.hpp:26). val: stored precision to use;
+ * vec: precision of x and y; acc: accumulation (LA_F64 only with FP64
+ * values and vectors). */
+int la_spmv(la_mat_t A, la_spmv_alg alg, la_dtype val, la_dtype vec,
+            la_dtype acc, double alpha, const void *x, double beta, void *y);
+/* y = alpha*A*x + beta*z with a separate input z (residual form b - A x). */
+int la_spmv_axpby(la_mat_t A, la_spmv_alg alg, la_dtype val, la_dtype vec,
+                  la_dtype acc, double alpha, const void *x, double beta,
+                  const void *z, void *y);
+/* y = alpha*A^T*x + beta*y: mkl_sparse_mat::transpose_mult_vec
+ * (mkl_sparse_mat.cpp:331-336); builds A^T on first use. */
+int la_spmv_transpose(la_mat_t A, la_dtype val, la_dtype vec, double alpha,
+                      const void *x, double beta, void *y);
+/* gradient of a loss w.r.t. the stored values: g[k] = -lambda[row k] *
+ * x[col k] for every stored entry (FP64, device). */
+int la_csr_value_grad(la_mat_t A, const double *lambda, const double *x,
+                      double *g);
+
+/* ====================== sparse triangular solve ======================== */
+typedef struct la_trsv_s *la_trsv_t;
+
+/* OptimizedTriangularSolve::analysis (triangle_solve.tpp:17-113) and the
+ * level sets of TopologicalSort2: (ai,aj,av) is a STRICT triangle, diag the
+ * diagonal (NULL = unit). Lower solves sweep forward
+ * (triangle_solve.hpp:32-43), upper backward (:59-71):
+ * x[i] = (b[i] - sum_j T_ij x_j) / diag[i]. */
+int la_trsv_create(int n, int base, const int *ai, const int *aj,
+                   const double *av, const double *diag, la_uplo uplo,
+                   la_trsv_t *out);
+int la_trsv_destroy(la_trsv_t T);
+int la_trsv_add_dtype(la_trsv_t T, la_dtype dt);
+int la_trsv_set_stream(la_trsv_t T, void *hip_stream);
+int la_trsv_info(la_trsv_t T, int *n, int *nlevels, int64_t *nnz);
+/* OptimizedTriangularSolve::operator()(b, x) (triangle_solve.tpp:115-125);
+ * b and x are distinct device vectors. */
+int la_trsv_solve(la_trsv_t T, la_trsv_mode mode, la_dtype val, la_dtype vec,
+                  const void *b, void *x);
+/* (T + D)^T x = b, ForwardSubstitutionT's problem (triangle_solve.hpp:
+ * 87-98) for the adjoint; builds the transposed plan on first use. */
+int la_trsv_solve_transpose(la_trsv_t T, la_trsv_mode mode, la_dtype val,
+                            la_dtype vec, const void *b, void *x);
+
+/* ====================== ILU(k) / IC(k) apply =========================== */
+typedef struct la_prec_s *la_prec_t;
+
+/* incomplete_lu_base::solve (mkl_wrapper/incomplete_lu.cpp:36-48): factor
+ * in combined LU storage (ILU(k) numeric output). */
+int la_prec_create_ilu(int n, int base, const int *fai, const int *faj,
+                       const double *fav, la_prec_t *out);
+/* incomplete_choleksy_base::solve (mkl_wrapper/incomplete_cholesky.cpp:
+ * 21-29): upper factor R incl. diagonal, M = R^T R. */
+int la_prec_create_ic(int n, int base, const int *rai, const int *raj,
+                      const double *rav, la_prec_t *out);
+int la_prec_destroy(la_prec_t P);
+int la_prec_add_dtype(la_prec_t P, la_dtype dt);
+int la_prec_set_stream(la_prec_t P, void *hip_stream);
+/* z = M^-1 r (device vectors). */
+int la_prec_apply(la_prec_t P, la_trsv_mode mode, la_dtype val, la_dtype vec,
+                  const void *r, void *z);
+/* z = M^-T r. */
+int la_prec_apply_transpose(la_prec_t P, la_trsv_mode mode, la_dtype val,
+                            la_dtype vec, const void *r, void *z);
+int la_prec_info(la_prec_t P, int *levels_lower, int *levels_upper);
+
+/* ====================== fused BLAS-1 (FP64 accumulation) =============== */
+int la_dot(int64_t n, la_dtype vec, const void *x, const void *y,
+           double *result_host, void *hip_stream);
+/* y += a*x; *result_host = <y, z> (z may be y) */
+int la_axpy_dot(int64_t n, la_dtype vec, double a, const void *x, void *y,
+                const void *z, double *result_host, void *hip_stream);
+
+/* ====================== Krylov solvers ================================= */
+typedef enum {
+  LA_PREC_FP64 = 0,       /* everything FP64                              */
+  LA_PREC_MIXED_F32 = 1,  /* inner solve: FP32 matrix/factors/vectors      */
+  LA_PREC_MIXED_BF16 = 2  /* inner solve: BF16 matrix+factors, FP32 vectors*/
+} la_precision;
+
+typedef enum {
+  LA_STOP_TRUE_RESIDUAL = 0, /* ||b-Ax||/||b|| every iteration, with an
+                                extra SpMV (mkl_solver.cpp:211-225)        */
+  LA_STOP_RECURRENCE = 1     /* recurrence residual, confirmed by a true
+                                residual before returning                  */
+} la_stop_rule;
+
+typedef struct {
+  double rel_tol;      /* mkl_iterative_solver::set_rel_tol (default 1e-8) */
+  double abs_tol;      /* set_abs_tol (default 0)                          */
+  int max_iters;       /* set_max_iters (default 1000)                     */
+  int restart;         /* FGMRES: set_restart_steps (default 30)           */
+  int stop_rule;       /* la_stop_rule                                      */
+  int precision;       /* la_precision                                     */
+  int trsv_mode;       /* la_trsv_mode for the preconditioner apply        */
+  int spmv_alg;        /* la_spmv_alg                                      */
+  double inner_tol;    /* mixed: inner relative reduction per refinement   */
+  int max_outer;       /* mixed: refinement steps                          */
+  int print_level;     /* set_print_level: 1 prints each residual          */
+} la_solver_opts;
+
+typedef struct {
+  int converged;
+  int iters;           /* total (inner) iterations                         */
+  int outer_iters;     /* refinement steps (mixed), 1 otherwise            */
+  double rel_residual; /* final true ||b-Ax||/||b|| in FP64                */
+  double seconds;      /* wall time of the device-resident loop            */
+} la_solver_result;
+
+void la_solver_opts_default(la_solver_opts *o);
+/* mkl_pcg_solver::solve (mkl_wrapper/mkl_solver.cpp:150-251). b, x device
+ * FP64 vectors; x holds x0 on entry. P may be NULL. */
+int la_pcg_solve(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
+                 const double *b, double *x, la_solver_result *res);
+/* mkl_fgmres_solver::solve (mkl_wrapper/mkl_solver.cpp:253-386): right
+ * (flexible) preconditioned GMRES(restart). */
+int la_fgmres_solve(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
+                    const double *b, double *x, la_solver_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

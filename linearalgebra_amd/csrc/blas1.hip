@@ -1,0 +1,396 @@
+// Vector kernels of the Krylov loops: FP64-accumulated dot products
+// (deterministic two-stage reduction), fused CG updates with device-side
+// scalars, multi-dot / multi-axpy for GMRES orthogonalisation, precision
+// conversions.
+#include "dev_util.hpp"
+#include "device.hpp"
+
+namespace laamd {
+using namespace dev;
+
+namespace {
+constexpr int kB = 256;
+
+template <typename V>
+__device__ __forceinline__ double block_reduce(double v, double *sh) {
+  v = wave_sum(v);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kB / 64; ++w) t += sh[w];
+  return t;  // valid in thread 0
+}
+
+template <typename V>
+__global__ __launch_bounds__(kB) void k_dot_partial(int64_t n, const V *__restrict__ x,
+                                                    const V *__restrict__ y, double *part) {
+  __shared__ double sh[kB / 64];
+  double acc = 0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB)
+    acc += static_cast<double>(x[i]) * static_cast<double>(y[i]);
+  const double t = block_reduce<V>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// out[i] = sum_b part[i*nb + b], fixed order
+__global__ void k_reduce_partials(int k, int nb, const double *__restrict__ part, double *out) {
+  __shared__ double sh[kB / 64];
+  for (int i = 0; i < k; ++i) {
+    double acc = 0;
+    for (int b = threadIdx.x; b < nb; b += kB) acc += part[static_cast<int64_t>(i) * nb + b];
+    acc = wave_sum(acc);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sh[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0;
+      for (int w = 0; w < kB / 64; ++w) t += sh[w];
+      out[i] = t;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kB) void k_multidot_partial(int64_t n, int k,
+                                                         const V *__restrict__ Vm, int64_t ld,
+                                                         const V *__restrict__ w,
+                                                         double *part) {
+  __shared__ double sh[kB / 64];
+  // one pass per basis vector; w is re-read from L2
+  for (int i = 0; i < k; ++i) {
+    const V *v = Vm + static_cast<int64_t>(i) * ld;
+    double acc = 0;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+         t += static_cast<int64_t>(gridDim.x) * kB)
+      acc += static_cast<double>(v[t]) * static_cast<double>(w[t]);
+    const double s = block_reduce<V>(acc, sh);
+    if (threadIdx.x == 0) part[static_cast<int64_t>(i) * gridDim.x + blockIdx.x] = s;
+    __syncthreads();
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kB) void k_multiaxpy(int64_t n, int k, const V *__restrict__ Vm,
+                                                  int64_t ld, const double *__restrict__ h,
+                                                  V *w) {
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB) {
+    double acc = static_cast<double>(w[t]);
+    for (int i = 0; i < k; ++i) acc -= h[i] * static_cast<double>(Vm[static_cast<int64_t>(i) * ld + t]);
+    w[t] = static_cast<V>(acc);
+  }
+}
+
+template <typename V>
+__global__ void k_axpby(int64_t n, double a, const V *__restrict__ x, double b, V *y) {
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB) {
+    const double yv = b == 0.0 ? 0.0 : b * static_cast<double>(y[t]);
+    y[t] = static_cast<V>(a * static_cast<double>(x[t]) + yv);
+  }
+}
+
+template <typename S, typename D>
+__global__ void k_copy(int64_t n, const S *__restrict__ s, D *d, double scale) {
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB)
+    d[t] = static_cast<D>(static_cast<double>(s[t]) * scale);
+}
+
+template <typename V>
+__global__ void k_zero(int64_t n, V *x) {
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB)
+    x[t] = V(0);
+}
+
+// x += alpha p ; r -= alpha q with alpha = sc[0] / sc[1]; partial <r,r>
+template <typename V>
+__global__ __launch_bounds__(kB) void k_cg_update_xr(int64_t n, V *x, V *r, const V *__restrict__ p,
+                                                     const V *__restrict__ q,
+                                                     const double *__restrict__ sc,
+                                                     double *part) {
+  __shared__ double sh[kB / 64];
+  const double alpha = sc[0] / sc[1];
+  const V a = static_cast<V>(alpha);
+  double acc = 0;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB) {
+    x[t] += a * p[t];
+    const V rn = r[t] - a * q[t];
+    r[t] = rn;
+    acc += static_cast<double>(rn) * static_cast<double>(rn);
+  }
+  const double s = block_reduce<V>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// p = z + beta p, beta = sc[3] / sc[0]; thread 0 of block 0 cannot move
+// sc[3] into sc[0] here (other blocks still read sc[0]), the host enqueues
+// that copy after this kernel.
+template <typename V>
+__global__ void k_cg_update_p(int64_t n, V *p, const V *__restrict__ z,
+                              const double *__restrict__ sc) {
+  const double beta = sc[3] / sc[0];
+  const V b = static_cast<V>(beta);
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB)
+    p[t] = z[t] + b * p[t];
+}
+
+__global__ void k_move_scalar(double *sc, int dst, int src) { sc[dst] = sc[src]; }
+
+template <typename V>
+__global__ void k_add_scaled(int64_t n, const V *__restrict__ d, double scale, double *x) {
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB)
+    x[t] += scale * static_cast<double>(d[t]);
+}
+
+template <typename V>
+__global__ void k_scale_by_inv(int64_t n, const V *__restrict__ w, const double *__restrict__ h,
+                               V *v) {
+  const double inv = 1.0 / h[0];
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB)
+    v[t] = static_cast<V>(static_cast<double>(w[t]) * inv);
+}
+
+// FP64 -> FP32 / BF16 (round to nearest even) on the device
+__global__ void k_f64_to_f32(int64_t n, const double *__restrict__ s, float *d) {
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB)
+    d[t] = static_cast<float>(s[t]);
+}
+__device__ __forceinline__ uint16_t dev_f64_to_bf16(double v) {
+  // round the double directly at BF16 precision (7 stored mantissa bits):
+  // add half an ulp of bit 45 with ties-to-even, truncate, narrow exactly
+  unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned long long exp = (u >> 52) & 0x7FF;
+  if (exp == 0x7FF) {  // inf / nan
+    const uint32_t f = __float_as_uint(static_cast<float>(v));
+    return static_cast<uint16_t>((f >> 16) | ((f & 0xFFFF) ? 0x40 : 0));
+  }
+  if (exp < 1023 - 126 || exp > 1023 + 127) {  // outside float normals
+    const uint32_t f = __float_as_uint(static_cast<float>(v));
+    const uint32_t r = f + 0x7FFF + ((f >> 16) & 1);
+    return static_cast<uint16_t>(r >> 16);
+  }
+  u += (1ULL << 44) - 1 + ((u >> 45) & 1);
+  u &= ~((1ULL << 45) - 1);
+  const uint32_t f = __float_as_uint(static_cast<float>(__longlong_as_double(static_cast<long long>(u))));
+  return static_cast<uint16_t>(f >> 16);
+}
+__global__ void k_f64_to_bf16(int64_t n, const double *__restrict__ s, bf16 *d) {
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB)
+    d[t].bits = dev_f64_to_bf16(s[t]);
+}
+
+inline unsigned grid_for(int64_t n, int64_t cap = 4096) {
+  int64_t g = (n + kB - 1) / kB;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return static_cast<unsigned>(g);
+}
+}  // namespace
+
+// ------------------------------------------------------------------ host
+uint16_t double_to_bf16_rne(double d) {
+  uint64_t u;
+  std::memcpy(&u, &d, 8);
+  const uint64_t exp = (u >> 52) & 0x7FF;
+  auto via_float = [](double v) {
+    float f = static_cast<float>(v);
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    if ((b & 0x7F800000U) == 0x7F800000U) return static_cast<uint16_t>((b >> 16) | ((b & 0xFFFF) ? 0x40 : 0));
+    return static_cast<uint16_t>((b + 0x7FFF + ((b >> 16) & 1)) >> 16);
+  };
+  if (exp == 0x7FF || exp < 1023 - 126 || exp > 1023 + 127) return via_float(d);
+  u += (1ULL << 44) - 1 + ((u >> 45) & 1);
+  u &= ~((1ULL << 45) - 1);
+  double r;
+  std::memcpy(&r, &u, 8);
+  float f = static_cast<float>(r);  // exact: r has at most 8 significant bits
+  uint32_t b;
+  std::memcpy(&b, &f, 4);
+  return static_cast<uint16_t>(b >> 16);
+}
+
+float bf16_to_float_host(uint16_t b) {
+  const uint32_t u = static_cast<uint32_t>(b) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+void launch_convert_f64(const double *src, int64_t n, la_dtype dst_dt, void *dst, hipStream_t s) {
+  if (n == 0) return;
+  if (dst_dt == LA_F32)
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n)), dim3(kB), 0, s, n, src, static_cast<float *>(dst));
+  else if (dst_dt == LA_BF16)
+    hipLaunchKernelGGL(k_f64_to_bf16, dim3(grid_for(n)), dim3(kB), 0, s, n, src, static_cast<bf16 *>(dst));
+  else
+    LA_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  LA_HIP(hipGetLastError());
+}
+
+void MultiValues::set_from_device_f64(const double *d, int64_t cnt, hipStream_t s) {
+  count = cnt;
+  if (!has[LA_F64]) {
+    v[LA_F64].alloc(sizeof(double) * cnt);
+    has[LA_F64] = true;
+  }
+  if (cnt && d != v[LA_F64].p)
+    LA_HIP(hipMemcpyAsync(v[LA_F64].p, d, sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
+  for (int dt = 1; dt < 3; ++dt)
+    if (has[dt]) launch_convert_f64(v[LA_F64].as<double>(), cnt, static_cast<la_dtype>(dt), v[dt].p, s);
+}
+
+void MultiValues::set_from_host(const double *h, int64_t cnt, hipStream_t s) {
+  count = cnt;
+  if (!has[LA_F64]) {
+    v[LA_F64].alloc(sizeof(double) * cnt);
+    has[LA_F64] = true;
+  }
+  v[LA_F64].upload(h, sizeof(double) * cnt, s);
+  for (int dt = 1; dt < 3; ++dt)
+    if (has[dt]) launch_convert_f64(v[LA_F64].as<double>(), cnt, static_cast<la_dtype>(dt), v[dt].p, s);
+}
+
+void MultiValues::add(la_dtype dt, hipStream_t s) {
+  if (has[dt]) return;
+  if (!has[LA_F64]) throw la_error(LA_ERR_STATE, "no FP64 values to convert from");
+  v[dt].alloc(dtype_size(dt) * static_cast<size_t>(count));
+  launch_convert_f64(v[LA_F64].as<double>(), count, dt, v[dt].p, s);
+  has[dt] = true;
+}
+
+Scratch::Scratch() {
+  partials.alloc(sizeof(double) * kRedBlocks * kMaxDots);
+  scalars.alloc(sizeof(double) * 256);
+  LA_HIP(hipHostMalloc(reinterpret_cast<void **>(&host), sizeof(double) * 256, hipHostMallocDefault));
+}
+Scratch::~Scratch() {
+  if (host) (void)hipHostFree(host);
+}
+
+void dot_dev(int64_t n, la_dtype vec, const void *x, const void *y, double *partials, double *out,
+             hipStream_t s) {
+  if (vec == LA_F64)
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                       static_cast<const double *>(x), static_cast<const double *>(y), partials);
+  else
+    hipLaunchKernelGGL(k_dot_partial<float>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                       static_cast<const float *>(x), static_cast<const float *>(y), partials);
+  LA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kB), 0, s, 1, kRedBlocks, partials, out);
+  LA_HIP(hipGetLastError());
+}
+
+void multidot_dev(int64_t n, int k, la_dtype vec, const void *V, int64_t ld, const void *w,
+                  double *partials, double *out, hipStream_t s) {
+  LA_CHECK_ARG(k <= kMaxDots, "too many dot products");
+  if (vec == LA_F64)
+    hipLaunchKernelGGL(k_multidot_partial<double>, dim3(kRedBlocks), dim3(kB), 0, s, n, k,
+                       static_cast<const double *>(V), ld, static_cast<const double *>(w), partials);
+  else
+    hipLaunchKernelGGL(k_multidot_partial<float>, dim3(kRedBlocks), dim3(kB), 0, s, n, k,
+                       static_cast<const float *>(V), ld, static_cast<const float *>(w), partials);
+  LA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kB), 0, s, k, kRedBlocks, partials, out);
+  LA_HIP(hipGetLastError());
+}
+
+void multiaxpy_dev(int64_t n, int k, la_dtype vec, const void *V, int64_t ld, const double *h,
+                   void *w, hipStream_t s) {
+  if (vec == LA_F64)
+    hipLaunchKernelGGL(k_multiaxpy<double>, dim3(grid_for(n)), dim3(kB), 0, s, n, k,
+                       static_cast<const double *>(V), ld, h, static_cast<double *>(w));
+  else
+    hipLaunchKernelGGL(k_multiaxpy<float>, dim3(grid_for(n)), dim3(kB), 0, s, n, k,
+                       static_cast<const float *>(V), ld, h, static_cast<float *>(w));
+  LA_HIP(hipGetLastError());
+}
+
+void axpby(int64_t n, la_dtype vec, double a, const void *x, double b, void *y, hipStream_t s) {
+  if (vec == LA_F64)
+    hipLaunchKernelGGL(k_axpby<double>, dim3(grid_for(n)), dim3(kB), 0, s, n, a,
+                       static_cast<const double *>(x), b, static_cast<double *>(y));
+  else
+    hipLaunchKernelGGL(k_axpby<float>, dim3(grid_for(n)), dim3(kB), 0, s, n, a,
+                       static_cast<const float *>(x), b, static_cast<float *>(y));
+  LA_HIP(hipGetLastError());
+}
+
+void copy_vec(int64_t n, la_dtype sd, const void *src, la_dtype dd, void *dst, double scale,
+              hipStream_t s) {
+  const unsigned g = grid_for(n);
+  if (sd == LA_F64 && dd == LA_F64)
+    hipLaunchKernelGGL((k_copy<double, double>), dim3(g), dim3(kB), 0, s, n, static_cast<const double *>(src), static_cast<double *>(dst), scale);
+  else if (sd == LA_F64 && dd == LA_F32)
+    hipLaunchKernelGGL((k_copy<double, float>), dim3(g), dim3(kB), 0, s, n, static_cast<const double *>(src), static_cast<float *>(dst), scale);
+  else if (sd == LA_F32 && dd == LA_F64)
+    hipLaunchKernelGGL((k_copy<float, double>), dim3(g), dim3(kB), 0, s, n, static_cast<const float *>(src), static_cast<double *>(dst), scale);
+  else
+    hipLaunchKernelGGL((k_copy<float, float>), dim3(g), dim3(kB), 0, s, n, static_cast<const float *>(src), static_cast<float *>(dst), scale);
+  LA_HIP(hipGetLastError());
+}
+
+void fill_zero(int64_t n, la_dtype vec, void *x, hipStream_t s) {
+  if (vec == LA_F64)
+    hipLaunchKernelGGL(k_zero<double>, dim3(grid_for(n)), dim3(kB), 0, s, n, static_cast<double *>(x));
+  else
+    hipLaunchKernelGGL(k_zero<float>, dim3(grid_for(n)), dim3(kB), 0, s, n, static_cast<float *>(x));
+  LA_HIP(hipGetLastError());
+}
+
+void cg_update_xr(int64_t n, la_dtype vec, void *x, void *r, const void *p, const void *q,
+                  const double *sc, double *partials, double *rr_out, hipStream_t s) {
+  if (vec == LA_F64)
+    hipLaunchKernelGGL(k_cg_update_xr<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                       static_cast<double *>(x), static_cast<double *>(r),
+                       static_cast<const double *>(p), static_cast<const double *>(q), sc, partials);
+  else
+    hipLaunchKernelGGL(k_cg_update_xr<float>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                       static_cast<float *>(x), static_cast<float *>(r),
+                       static_cast<const float *>(p), static_cast<const float *>(q), sc, partials);
+  LA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kB), 0, s, 1, kRedBlocks, partials, rr_out);
+  LA_HIP(hipGetLastError());
+}
+
+void cg_update_p(int64_t n, la_dtype vec, void *p, const void *z, double *sc, hipStream_t s) {
+  if (vec == LA_F64)
+    hipLaunchKernelGGL(k_cg_update_p<double>, dim3(grid_for(n)), dim3(kB), 0, s, n,
+                       static_cast<double *>(p), static_cast<const double *>(z), sc);
+  else
+    hipLaunchKernelGGL(k_cg_update_p<float>, dim3(grid_for(n)), dim3(kB), 0, s, n,
+                       static_cast<float *>(p), static_cast<const float *>(z), sc);
+  LA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_move_scalar, dim3(1), dim3(1), 0, s, sc, 0, 3);
+  LA_HIP(hipGetLastError());
+}
+
+void add_scaled(int64_t n, la_dtype dvec, const void *d, double scale, double *x64, hipStream_t s) {
+  if (dvec == LA_F64)
+    hipLaunchKernelGGL(k_add_scaled<double>, dim3(grid_for(n)), dim3(kB), 0, s, n, static_cast<const double *>(d), scale, x64);
+  else
+    hipLaunchKernelGGL(k_add_scaled<float>, dim3(grid_for(n)), dim3(kB), 0, s, n, static_cast<const float *>(d), scale, x64);
+  LA_HIP(hipGetLastError());
+}
+
+void scale_by_inv(int64_t n, la_dtype vec, const void *w, const double *h, void *v, hipStream_t s) {
+  if (vec == LA_F64)
+    hipLaunchKernelGGL(k_scale_by_inv<double>, dim3(grid_for(n)), dim3(kB), 0, s, n, static_cast<const double *>(w), h, static_cast<double *>(v));
+  else
+    hipLaunchKernelGGL(k_scale_by_inv<float>, dim3(grid_for(n)), dim3(kB), 0, s, n, static_cast<const float *>(w), h, static_cast<float *>(v));
+  LA_HIP(hipGetLastError());
+}
+
+}  // namespace laamd

@@ -1,0 +1,132 @@
+// Device-side objects of the hot path (matrix, triangular-solve plan,
+// preconditioner) and the launchers the host code calls.
+#pragma once
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "common.hpp"
+
+namespace laamd {
+
+// -------------------------------------------------------------- CSR matrix
+struct Mat {
+  int rows = 0, cols = 0;
+  int64_t nnz = 0;
+  DevBuf ai, aj;       // 0-based int32 row pointers / columns
+  MultiValues vals;    // stored values in up to three precisions
+  hipStream_t stream = nullptr;
+  double avg_row = 0;  // nnz / rows
+  int max_row = 0;
+  // nnz-balanced segment table (ALBUS / SEGSUM)
+  bool seg_ready = false;
+  int64_t nseg = 0;
+  DevBuf seg_first_row, seg_head, seg_tail, seg_tail_row;
+  // A^T, built on first transposed product
+  std::unique_ptr<Mat> transposed;
+  // host copy of the pattern, kept for the transpose and the value map
+  std::vector<int> h_ai, h_aj;
+};
+
+std::unique_ptr<Mat> make_mat(int rows, int cols, int base, const int *ai, const int *aj,
+                              const double *av, hipStream_t s);
+void spmv_prepare_segments(Mat &A);
+// y = alpha*A*x + beta*z (z may equal y; beta == 0 never reads z)
+void spmv(Mat &A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, double alpha,
+          const void *x, double beta, const void *z, void *y, hipStream_t s);
+void spmv_transpose(Mat &A, la_dtype val, la_dtype vec, double alpha, const void *x,
+                    double beta, void *y, hipStream_t s);
+void csr_value_grad(Mat &A, const double *lambda, const double *x, double *g, hipStream_t s);
+
+// -------------------------------------------------------------- SpTRSV plan
+struct TrsvPlan {
+  int n = 0, nlevels = 0;
+  int64_t nnz = 0;
+  bool upper = false, unit = true;
+  int G = 1;                   // lanes per row
+  DevBuf rowptr, col, perm;    // rows in level order; columns keep original ids
+  MultiValues val, diag;       // values / diagonal in level order
+  std::vector<int> level_ptr;  // host, nlevels+1
+  int nchunks = 0;
+  DevBuf chunk_start, counter, err_flag;
+  hipStream_t stream = nullptr;
+  // host copy of the strict triangle + diagonal (0-based), for the transpose
+  std::vector<int> h_ai, h_aj;
+  std::vector<double> h_av, h_diag;
+  std::unique_ptr<TrsvPlan> transposed;
+  // captured per-level launch sequences, keyed by (val, vec, b, x)
+  std::map<std::tuple<int, int, const void *, void *>, hipGraphExec_t> graphs;
+  ~TrsvPlan();
+};
+
+std::unique_ptr<TrsvPlan> make_trsv(int n, int base, const int *ai, const int *aj,
+                                    const double *av, const double *diag, bool upper,
+                                    hipStream_t s);
+void trsv_add_dtype(TrsvPlan &T, la_dtype dt);
+void trsv_solve(TrsvPlan &T, la_trsv_mode mode, la_dtype val, la_dtype vec, const void *b,
+                void *x, hipStream_t s);
+TrsvPlan &trsv_transposed(TrsvPlan &T);
+// checks (and clears) the sync-free spin-bound error flag; synchronizes s
+bool trsv_check_timeout(TrsvPlan &T, hipStream_t s);
+
+// -------------------------------------------------------------- ILU / IC
+struct Prec {
+  bool ic = false;
+  int n = 0;
+  std::unique_ptr<TrsvPlan> lower, upper;
+  DevBuf tmp[2];  // intermediate vector, FP64 and FP32
+  hipStream_t stream = nullptr;
+};
+std::unique_ptr<Prec> make_prec_ilu(int n, int base, const int *fai, const int *faj,
+                                    const double *fav, hipStream_t s);
+std::unique_ptr<Prec> make_prec_ic(int n, int base, const int *rai, const int *raj,
+                                   const double *rav, hipStream_t s);
+void prec_add_dtype(Prec &P, la_dtype dt);
+void prec_apply(Prec &P, la_trsv_mode mode, la_dtype val, la_dtype vec, const void *r, void *z,
+                hipStream_t s);
+void prec_apply_transpose(Prec &P, la_trsv_mode mode, la_dtype val, la_dtype vec,
+                          const void *r, void *z, hipStream_t s);
+
+// -------------------------------------------------------------- BLAS-1
+// Deterministic FP64-accumulated reductions: a fixed grid of kRedBlocks
+// blocks writes per-block partials, a one-block kernel sums them in order.
+constexpr int kRedBlocks = 512;
+struct Scratch {
+  DevBuf partials;  // kRedBlocks * kMaxDots doubles
+  DevBuf scalars;   // device scalars used by the solvers
+  double *host = nullptr;  // pinned mirror of a few scalars
+  Scratch();
+  ~Scratch();
+};
+constexpr int kMaxDots = 64;
+
+// out[0] = <x,y>
+void dot_dev(int64_t n, la_dtype vec, const void *x, const void *y, double *partials,
+             double *out, hipStream_t s);
+// out[i] = <V_i, w> for i < k; V is k vectors of length n, stride ld
+void multidot_dev(int64_t n, int k, la_dtype vec, const void *V, int64_t ld, const void *w,
+                  double *partials, double *out, hipStream_t s);
+// w -= sum_i h[i] V_i (h device FP64)
+void multiaxpy_dev(int64_t n, int k, la_dtype vec, const void *V, int64_t ld, const double *h,
+                   void *w, hipStream_t s);
+// y = a*x + b*y with host scalars
+void axpby(int64_t n, la_dtype vec, double a, const void *x, double b, void *y,
+           hipStream_t s);
+void copy_vec(int64_t n, la_dtype src_dt, const void *src, la_dtype dst_dt, void *dst,
+              double scale, hipStream_t s);
+void fill_zero(int64_t n, la_dtype vec, void *x, hipStream_t s);
+
+// CG step kernels with device scalars (sc = scalars array)
+//   sc[0] = rz, sc[1] = pq, sc[2] = rr (or true residual), sc[3] = rz_new
+void cg_update_xr(int64_t n, la_dtype vec, void *x, void *r, const void *p, const void *q,
+                  const double *sc, double *partials, double *rr_out, hipStream_t s);
+void cg_update_p(int64_t n, la_dtype vec, void *p, const void *z, double *sc, hipStream_t s);
+// x64 += scale * d (d in vec precision)
+void add_scaled(int64_t n, la_dtype dvec, const void *d, double scale, double *x64,
+                hipStream_t s);
+// v = w / h (h device scalar) ; used by GMRES
+void scale_by_inv(int64_t n, la_dtype vec, const void *w, const double *h, void *v,
+                  hipStream_t s);
+
+}  // namespace laamd

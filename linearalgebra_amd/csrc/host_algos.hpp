@@ -1,0 +1,51 @@
+// Host-side (CPU) algorithms of the hot path; see host_algos.cpp.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace laamd {
+
+int topo_sort2(int n, int base, const int *ai, const int *aj, bool upper,
+               int *iperm, int *prefix);
+void split_ldu(int n, int base, const int *ai, const int *aj, const double *av,
+               int *Lai, int *Laj, double *Lav, double *D, int *Uai, int *Uaj,
+               double *Uav, int64_t *lnnz, int64_t *unnz);
+void split_triangle(int n, int base, const int *ai, const int *aj,
+                    const double *av, bool upper, int *Tai, int *Taj,
+                    double *Tav, int64_t *tnnz);
+bool diagonal(int n, int base, const int *ai, const int *aj, const double *av,
+              int *diagpos, double *diag);
+void transpose(int rows, int cols, int base, const int *ai, const int *aj,
+               const double *av, int *ti, int *tj, double *tv);
+void permute(int n, int base, const int *ai, const int *aj, const double *av,
+             const int *iperm, const int *perm, int *pai, int *paj,
+             double *pav);
+void rcm(int n, int base, const int *ai, const int *aj, int *iperm, int *perm);
+void iluk_symbolic(int n, int base, const int *ai, const int *aj, int level,
+                   std::vector<int> &fai, std::vector<int> &faj,
+                   std::vector<int> &first_upper);
+bool iluk_numeric(int n, int base, const int *ai, const int *aj,
+                  const double *av, const int *fai, const int *faj,
+                  const int *first_upper, double *fav);
+void icc_symbolic(int n, int base, const int *ai, const int *aj,
+                  const int *diag_pos, int lvl, std::vector<int> &icc_ai,
+                  std::vector<int> &icc_aj);
+bool icc_numeric(int n, int base, const int *ai, const int *aj, const double *av,
+                 const int *diag_pos, double omega, const int *icc_ai,
+                 const int *icc_aj, double *icc_av);
+
+// seeded generators (gen.cpp)
+struct SplitMix64 {
+  uint64_t s;
+  explicit SplitMix64(uint64_t seed) : s(seed) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }
+  // uniform in [0,1) with 53 random bits
+  double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+}  // namespace laamd

@@ -1,0 +1,304 @@
+// Preconditioned Krylov loops driving the device kernels:
+//  * PCG (the loop of mkl_pcg_solver::solve, mkl_wrapper/mkl_solver.cpp:
+//    150-251): textbook PCG (Saad, Alg. 9.1); the reference's stopping test
+//    is the true residual ||b - A x|| / ||b|| (or < abs_tol) evaluated every
+//    iteration with an extra SpMV (:211-225).
+//  * FGMRES(m) (mkl_fgmres_solver::solve, :253-386): right/flexible
+//    preconditioning, classical Gram-Schmidt with one re-orthogonalisation
+//    pass (one multi-dot + one multi-axpy per pass), Givens rotations on
+//    the host; stops on the Arnoldi residual estimate / ||b|| (:335-346) or
+//    a Hessenberg breakdown h_{k+1,k} < 1e-16 (:353-367).
+// Mixed precision: the inner solve runs with FP32 vectors and FP32/BF16
+// matrix and factors (FP32 accumulation, FP64 dot products); an outer FP64
+// iterative refinement computes r = b - A x with the FP64 matrix, solves
+// A d = r / ||r|| approximately, and updates x += ||r|| d in FP64.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "device.hpp"
+
+namespace laamd {
+
+namespace {
+struct Work {
+  int64_t n = 0;
+  int m = 0;
+  DevBuf v64[6], v32[6];
+  DevBuf V, Z;  // GMRES bases
+  la_dtype basis_dt = LA_F64;
+  Scratch sc;
+};
+
+thread_local std::unique_ptr<Work> tl_work;
+
+Work &work_for(int64_t n) {
+  if (!tl_work || tl_work->n != n) {
+    tl_work = std::make_unique<Work>();
+    tl_work->n = n;
+    for (int i = 0; i < 6; ++i) {
+      tl_work->v64[i].alloc(sizeof(double) * std::max<int64_t>(n, 1));
+      tl_work->v32[i].alloc(sizeof(float) * std::max<int64_t>(n, 1));
+    }
+  }
+  return *tl_work;
+}
+
+double fetch(Scratch &S, const double *dev, hipStream_t s) {
+  LA_HIP(hipMemcpyAsync(S.host, dev, sizeof(double), hipMemcpyDeviceToHost, s));
+  LA_HIP(hipStreamSynchronize(s));
+  return S.host[0];
+}
+
+double norm2(int64_t n, la_dtype dt, const void *x, Scratch &S, hipStream_t s) {
+  double *d = S.scalars.as<double>() + 200;
+  dot_dev(n, dt, x, x, S.partials.as<double>(), d, s);
+  return std::sqrt(fetch(S, d, s));
+}
+
+la_dtype inner_val_dtype(int precision) {
+  return precision == LA_PREC_MIXED_BF16 ? LA_BF16 : LA_F32;
+}
+
+void apply_M(Prec *P, const la_solver_opts &o, la_dtype val, la_dtype vec, const void *r,
+             void *z, int64_t n, hipStream_t s) {
+  if (P) prec_apply(*P, static_cast<la_trsv_mode>(o.trsv_mode), val, vec, r, z, s);
+  else copy_vec(n, vec, r, vec, z, 1.0, s);
+}
+
+// One PCG run on (val, vec) precision. x holds x0; vectors b/x in `vec`.
+// In `true_residual` mode the stop test uses ||b - A x|| (FP64 vectors
+// only); otherwise the recurrence residual, relative to ||b||.
+struct CgResult {
+  bool converged = false;
+  int iters = 0;
+  double last_res = 0;
+};
+
+CgResult pcg_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_dtype vec,
+                  const void *b, void *x, double nb, double rel_tol, double abs_tol,
+                  int maxiter, bool true_residual, Work &W, hipStream_t s) {
+  const int64_t n = A.rows;
+  const la_dtype acc = (val == LA_F64 && vec == LA_F64) ? LA_F64 : LA_F32;
+  DevBuf *vb = vec == LA_F64 ? W.v64 : W.v32;
+  void *r = vb[0].p, *z = vb[1].p, *p = vb[2].p, *q = vb[3].p, *t = vb[4].p;
+  double *sc = W.sc.scalars.as<double>(), *part = W.sc.partials.as<double>();
+  const la_spmv_alg alg = static_cast<la_spmv_alg>(o.spmv_alg);
+  CgResult R;
+  // r = b - A x ; z = M r ; p = z ; rz = <r,z>
+  spmv(A, alg, val, vec, acc, -1.0, x, 1.0, b, r, s);
+  apply_M(P, o, val, vec, r, z, n, s);
+  copy_vec(n, vec, z, vec, p, 1.0, s);
+  dot_dev(n, vec, r, z, part, sc + 0, s);
+  for (int it = 1; it <= maxiter; ++it) {
+    spmv(A, alg, val, vec, acc, 1.0, p, 0.0, q, q, s);
+    dot_dev(n, vec, p, q, part, sc + 1, s);
+    cg_update_xr(n, vec, x, r, p, q, sc, part, sc + 2, s);
+    if (true_residual) {
+      spmv(A, alg, val, vec, acc, -1.0, x, 1.0, b, t, s);
+      dot_dev(n, vec, t, t, part, sc + 2, s);
+    }
+    const double res = std::sqrt(fetch(W.sc, sc + 2, s));
+    R.iters = it;
+    R.last_res = res;
+    if (o.print_level == 1) std::fprintf(stderr, "%3d = %lg/%lg=%lg\n", it, res, nb, res / nb);
+    if (res / nb < rel_tol || res < abs_tol || !std::isfinite(res)) {
+      R.converged = std::isfinite(res);
+      break;
+    }
+    apply_M(P, o, val, vec, r, z, n, s);
+    dot_dev(n, vec, r, z, part, sc + 3, s);
+    cg_update_p(n, vec, p, z, sc, s);
+  }
+  return R;
+}
+
+// One FGMRES run (restart cycles until converged or maxiter) on (val, vec).
+CgResult fgmres_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_dtype vec,
+                     const void *b, void *x, double nb, double rel_tol, double abs_tol,
+                     int maxiter, int m, Work &W, hipStream_t s) {
+  const int64_t n = A.rows;
+  const la_dtype acc = (val == LA_F64 && vec == LA_F64) ? LA_F64 : LA_F32;
+  const size_t es = dtype_size(vec);
+  if (W.m < m || W.basis_dt != vec || !W.V.p) {
+    W.V.alloc(es * static_cast<size_t>(n) * (m + 1));
+    W.Z.alloc(es * static_cast<size_t>(n) * m);
+    W.m = m;
+    W.basis_dt = vec;
+  }
+  LA_CHECK_ARG(m + 1 <= kMaxDots, "restart too large");
+  char *V = static_cast<char *>(W.V.p), *Z = static_cast<char *>(W.Z.p);
+  auto Vj = [&](int j) { return static_cast<void *>(V + es * static_cast<size_t>(n) * j); };
+  auto Zj = [&](int j) { return static_cast<void *>(Z + es * static_cast<size_t>(n) * j); };
+  DevBuf *vb = vec == LA_F64 ? W.v64 : W.v32;
+  void *w = vb[0].p;
+  double *sc = W.sc.scalars.as<double>(), *part = W.sc.partials.as<double>();
+  double *h1 = sc + 8, *h2 = sc + 8 + kMaxDots, *hn = sc + 8 + 2 * kMaxDots;
+  const la_spmv_alg alg = static_cast<la_spmv_alg>(o.spmv_alg);
+  std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+  std::vector<double> hbuf(2 * kMaxDots + 1);
+  CgResult R;
+  int total = 0;
+  bool done = false;
+  while (!done) {
+    spmv(A, alg, val, vec, acc, -1.0, x, 1.0, b, w, s);
+    const double beta = norm2(n, vec, w, W.sc, s);
+    if (beta == 0.0 || beta / nb < rel_tol || beta < abs_tol) {
+      R.converged = true;
+      R.last_res = beta;
+      break;
+    }
+    copy_vec(n, vec, w, vec, Vj(0), 1.0 / beta, s);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;
+    for (int j = 0; j < m; ++j) {
+      apply_M(P, o, val, vec, Vj(j), Zj(j), n, s);
+      spmv(A, alg, val, vec, acc, 1.0, Zj(j), 0.0, w, w, s);
+      const int64_t ld = n;
+      multidot_dev(n, j + 1, vec, V, ld, w, part, h1, s);
+      multiaxpy_dev(n, j + 1, vec, V, ld, h1, w, s);
+      multidot_dev(n, j + 1, vec, V, ld, w, part, h2, s);
+      multiaxpy_dev(n, j + 1, vec, V, ld, h2, w, s);
+      dot_dev(n, vec, w, w, part, hn, s);
+      LA_HIP(hipMemcpyAsync(hbuf.data(), h1, sizeof(double) * (2 * kMaxDots + 1),
+                            hipMemcpyDeviceToHost, s));
+      LA_HIP(hipStreamSynchronize(s));
+      for (int i = 0; i <= j; ++i) H[static_cast<size_t>(i) * m + j] = hbuf[i] + hbuf[kMaxDots + i];
+      const double hnorm = std::sqrt(hbuf[2 * kMaxDots]);
+      H[static_cast<size_t>(j + 1) * m + j] = hnorm;
+      for (int i = 0; i < j; ++i) {
+        const double a = H[static_cast<size_t>(i) * m + j], c = H[static_cast<size_t>(i + 1) * m + j];
+        H[static_cast<size_t>(i) * m + j] = cs[i] * a + sn[i] * c;
+        H[static_cast<size_t>(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+      }
+      const double a = H[static_cast<size_t>(j) * m + j], c = H[static_cast<size_t>(j + 1) * m + j];
+      const double rr = std::sqrt(a * a + c * c);
+      cs[j] = rr == 0 ? 1.0 : a / rr;
+      sn[j] = rr == 0 ? 0.0 : c / rr;
+      H[static_cast<size_t>(j) * m + j] = rr;
+      H[static_cast<size_t>(j + 1) * m + j] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      ++total;
+      k = j + 1;
+      const double est = std::fabs(g[j + 1]);
+      R.last_res = est;
+      if (o.print_level == 1) std::fprintf(stderr, "%3d = %lg/%lg=%lg\n", total, est, nb, est / nb);
+      if (est / nb < rel_tol || est < abs_tol) { R.converged = true; done = true; break; }
+      if (hnorm < 1.0e-16) { R.converged = true; done = true; break; }
+      if (total >= maxiter) { done = true; break; }
+      copy_vec(n, vec, w, vec, Vj(j + 1), 1.0 / hnorm, s);
+    }
+    for (int i = k - 1; i >= 0; --i) {
+      double acc_y = g[i];
+      for (int l = i + 1; l < k; ++l) acc_y -= H[static_cast<size_t>(i) * m + l] * y[l];
+      y[i] = acc_y / H[static_cast<size_t>(i) * m + i];
+    }
+    // x += Z y  (multiaxpy subtracts, so pass -y)
+    for (int l = 0; l < k; ++l) hbuf[l] = -y[l];
+    LA_HIP(hipMemcpyAsync(h1, hbuf.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    multiaxpy_dev(n, k, vec, Z, n, h1, x, s);
+    LA_HIP(hipStreamSynchronize(s));
+  }
+  R.iters = total;
+  return R;
+}
+
+using clk = std::chrono::steady_clock;
+
+template <typename Core>
+int run_solver(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
+               la_solver_result *res, Core core) {
+  hipStream_t s = A.stream;
+  const int64_t n = A.rows;
+  Work &W = work_for(n);
+  la_solver_result out{};
+  const double nb = norm2(n, LA_F64, b, W.sc, s);
+  const auto t0 = clk::now();
+  if (nb == 0.0) {  // zero right-hand side: x = 0 solves it exactly
+    fill_zero(n, LA_F64, x, s);
+    LA_HIP(hipStreamSynchronize(s));
+    out.converged = 1;
+    out.outer_iters = 1;
+  } else if (o.precision == LA_PREC_FP64) {
+    CgResult R = core(LA_F64, LA_F64, static_cast<const void *>(b), static_cast<void *>(x), nb,
+                      o.rel_tol, o.abs_tol, o.max_iters, W);
+    out.iters = R.iters;
+    out.outer_iters = 1;
+    out.converged = R.converged;
+  } else {
+    const la_dtype vd = inner_val_dtype(o.precision);
+    if (!A.vals.has[vd]) A.vals.add(vd, s);
+    if (P) prec_add_dtype(*P, vd);
+    double *r64 = W.v64[5].as<double>();
+    float *rhs = W.v32[5].as<float>();
+    // the inner solution lives in a dedicated FP32 vector outside the ones
+    // the inner core uses (v32[0..4])
+    DevBuf d32(sizeof(float) * std::max<int64_t>(n, 1));
+    int total = 0, outer = 0;
+    bool conv = false;
+    for (outer = 0; outer < std::max(1, o.max_outer); ++outer) {
+      spmv(A, static_cast<la_spmv_alg>(o.spmv_alg), LA_F64, LA_F64, LA_F64, -1.0, x, 1.0, b, r64, s);
+      const double rn = norm2(n, LA_F64, r64, W.sc, s);
+      if (o.print_level == 1) std::fprintf(stderr, "outer %d: %lg/%lg=%lg\n", outer, rn, nb, rn / nb);
+      if (rn / nb < o.rel_tol || rn < o.abs_tol) { conv = true; break; }
+      if (total >= o.max_iters) break;
+      copy_vec(n, LA_F64, r64, LA_F32, rhs, 1.0 / rn, s);
+      fill_zero(n, LA_F32, d32.p, s);
+      CgResult R = core(vd, LA_F32, static_cast<const void *>(rhs), d32.p, 1.0, o.inner_tol, 0.0,
+                        o.max_iters - total, W);
+      total += R.iters;
+      add_scaled(n, LA_F32, d32.p, rn, x, s);
+    }
+    out.iters = total;
+    out.outer_iters = outer + (conv ? 0 : 0);
+    out.converged = conv;
+  }
+  LA_HIP(hipStreamSynchronize(s));
+  out.seconds = std::chrono::duration<double>(clk::now() - t0).count();
+  // final true residual in FP64
+  double *t = W.v64[4].as<double>();
+  spmv(A, LA_SPMV_AUTO, LA_F64, LA_F64, LA_F64, -1.0, x, 1.0, b, t, s);
+  const double tr = norm2(n, LA_F64, t, W.sc, s);
+  out.rel_residual = nb > 0 ? tr / nb : 0.0;
+  if (o.precision == LA_PREC_FP64 && out.converged && o.stop_rule == LA_STOP_RECURRENCE &&
+      !(out.rel_residual < o.rel_tol || tr < o.abs_tol * 1.0000001))
+    out.converged = 0;  // recurrence said yes, the true residual says no
+  if (P) {
+    if (trsv_check_timeout(*P->lower, s) || trsv_check_timeout(*P->upper, s))
+      throw la_error(LA_ERR_TIMEOUT, "sync-free triangular solve exceeded its spin bound");
+  }
+  if (res) *res = out;
+  return out.converged ? LA_OK : LA_ERR_NOT_CONVERGED;
+}
+}  // namespace
+
+int pcg_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
+              la_solver_result *res) {
+  LA_CHECK_ARG(A.rows == A.cols, "PCG needs a square matrix");
+  LA_CHECK_ARG(!P || P->n == A.rows, "preconditioner size mismatch");
+  hipStream_t s = A.stream;
+  return run_solver(A, P, o, b, x, res,
+                    [&](la_dtype val, la_dtype vec, const void *bb, void *xx, double nb,
+                        double rt, double at, int mi, Work &W) {
+                      const bool true_res = vec == LA_F64 && o.stop_rule == LA_STOP_TRUE_RESIDUAL;
+                      return pcg_core(A, P, o, val, vec, bb, xx, nb, rt, at, mi, true_res, W, s);
+                    });
+}
+
+int fgmres_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
+                 la_solver_result *res) {
+  LA_CHECK_ARG(A.rows == A.cols, "FGMRES needs a square matrix");
+  LA_CHECK_ARG(!P || P->n == A.rows, "preconditioner size mismatch");
+  hipStream_t s = A.stream;
+  const int m = o.restart > 0 ? o.restart : std::min(o.max_iters, kMaxDots - 1);
+  return run_solver(A, P, o, b, x, res,
+                    [&](la_dtype val, la_dtype vec, const void *bb, void *xx, double nb,
+                        double rt, double at, int mi, Work &W) {
+                      return fgmres_core(A, P, o, val, vec, bb, xx, nb, rt, at, mi, m, W, s);
+                    });
+}
+
+}  // namespace laamd

@@ -1,0 +1,340 @@
+// CSR SpMV on gfx950.
+//
+// Two kernel families (both compute y = alpha*A*x + beta*z):
+//  * row kernel: G lanes of a 64-wide wavefront per row, G the power of two
+//    at or above the mean row length; coalesced row streams, DPP reduction.
+//    Replaces SerialSPMV/ParallelSPMV (sparse_mat_op/spmv.hpp:12-57).
+//  * segment kernel: every wavefront owns a fixed slice of 64*E nonzeros
+//    (nnz-balanced, like ALBUSSPMV's equal nnz split, spmv.hpp:195-228).
+//    Products are staged in LDS; the rows that start inside the slice are
+//    reduced from LDS, the partial head/tail rows leave one partial each and
+//    a second tiny kernel adds them in slice order (the ALBUS boundary fix-up
+//    of spmv.hpp:275-292, deterministic). Replaces SegSumSPMV and ALBUSSPMV.
+#include <algorithm>
+
+#include "dev_util.hpp"
+#include "device.hpp"
+#include "host_algos.hpp"
+
+namespace laamd {
+using namespace dev;
+
+namespace {
+constexpr int kBlock = 256;
+constexpr int kSegE = 8;               // nonzeros per lane per segment
+constexpr int kSegNnz = 64 * kSegE;    // nonzeros per wavefront segment
+
+template <typename V>
+__device__ __forceinline__ V ldx(const V *x, int j) {
+  return x[j];
+}
+
+// ------------------------------------------------------------------ row
+template <int G, typename T, typename V, typename A>
+__global__ __launch_bounds__(kBlock) void k_spmv_row(int n, const int *__restrict__ ai,
+                                                     const int *__restrict__ aj,
+                                                     const T *__restrict__ av,
+                                                     const V *__restrict__ x, V *y,
+                                                     const V *z, A alpha, A beta) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t row = t / G;
+  const int sub = threadIdx.x & (G - 1);
+  if (row >= n) return;  // a whole group leaves together (G divides 64)
+  const int rs = ai[row], re = ai[row + 1];
+  A acc = 0;
+  for (int q = rs + sub; q < re; q += G) acc += A(ldm(av, q)) * A(ldx(x, ldi(aj, q)));
+  acc = group_sum<G>(acc);
+  if (sub == 0) {
+    A r = alpha * acc;
+    if (beta != A(0)) r += beta * A(z[row]);
+    y[row] = V(r);
+  }
+}
+
+// ------------------------------------------------------------------ segment
+// first_row[s] = first row r with ai[r] >= s*kSegNnz (r = n past the end);
+// first_row[nseg] = n.
+__global__ void k_seg_table(int n, int64_t nseg, const int *__restrict__ ai, int *first_row) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s > nseg) return;
+  if (s == nseg) { first_row[s] = n; return; }
+  const int64_t g = s * kSegNnz;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = lo + (hi - lo) / 2;
+    if (static_cast<int64_t>(ai[mid]) < g) lo = mid + 1; else hi = mid;
+  }
+  first_row[s] = lo;
+}
+
+// rows [rb, rb+R) owned by this segment, G lanes each, sums read from LDS
+template <int G, typename V, typename A>
+__device__ __forceinline__ void seg_rows(int lane, int rb, int R, int64_t g0, int64_t g1,
+                                         const int *__restrict__ ai, const A *p, V *y,
+                                         const V *z, A alpha, A beta, A *tail_part,
+                                         int64_t s) {
+  const int gi = lane / G, sub = lane & (G - 1);
+  if (gi >= R) return;  // whole groups only
+  const int r = rb + gi;
+  const int64_t rs = ai[r], re = ai[r + 1];
+  const int64_t qe = re < g1 ? re : g1;
+  A acc = 0;
+  for (int64_t q = rs + sub; q < qe; q += G) acc += p[q - g0];
+  acc = group_sum<G>(acc);
+  if (sub == 0) {
+    if (re <= g1) {
+      A o = alpha * acc;
+      if (beta != A(0)) o += beta * A(z[r]);
+      y[r] = V(o);
+    } else {
+      tail_part[s] = acc;  // continues into the next segment(s)
+    }
+  }
+}
+
+template <typename T, typename V, typename A>
+__global__ __launch_bounds__(kBlock) void k_spmv_seg(
+    int n, int64_t nnz, int64_t nseg, const int *__restrict__ ai, const int *__restrict__ aj,
+    const T *__restrict__ av, const V *__restrict__ x, const int *__restrict__ first_row,
+    A *head_part, A *tail_part, int *tail_row, V *y, const V *z, A alpha, A beta) {
+  __shared__ A sp[kBlock / 64][kSegNnz];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + wave;
+  const bool live = s < nseg;
+  const int64_t g0 = s * kSegNnz;
+  const int64_t g1 = live ? (g0 + kSegNnz < nnz ? g0 + kSegNnz : nnz) : g0;
+  A *p = sp[wave];
+#pragma unroll
+  for (int it = 0; it < kSegE; ++it) {
+    const int64_t g = g0 + it * 64 + lane;
+    A v = 0;
+    if (g < g1) v = A(ldm(av, g)) * A(ldx(x, ldi(aj, g)));
+    p[it * 64 + lane] = v;
+  }
+  __syncthreads();
+  if (!live) return;
+  const int f0 = first_row[s], f1 = first_row[s + 1];
+  // head: the row that started in an earlier segment (ALBUS bv[2t])
+  if (f0 > 0) {
+    const int64_t hb = ai[f0];
+    if (hb > g0) {
+      const int64_t he = hb < g1 ? hb : g1;
+      A acc = 0;
+      for (int64_t q = g0 + lane; q < he; q += 64) acc += p[q - g0];
+      acc = wave_sum(acc);
+      if (lane == 0) head_part[s] = acc;
+    }
+  }
+  if (lane == 0) tail_row[s] = (f1 > f0 && static_cast<int64_t>(ai[f1]) > g1) ? f1 - 1 : -1;
+  for (int rb = f0; rb < f1; rb += 64) {
+    const int R = (f1 - rb) < 64 ? (f1 - rb) : 64;
+    const int Gs = R > 32 ? 1 : R > 16 ? 2 : R > 8 ? 4 : R > 4 ? 8 : R > 2 ? 16 : R > 1 ? 32 : 64;
+    switch (Gs) {
+      case 1: seg_rows<1>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
+      case 2: seg_rows<2>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
+      case 4: seg_rows<4>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
+      case 8: seg_rows<8>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
+      case 16: seg_rows<16>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
+      case 32: seg_rows<32>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
+      default: seg_rows<64>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
+    }
+  }
+}
+
+// rows that cross segment boundaries: tail partial + the following head
+// partials, added in segment order
+template <typename V, typename A>
+__global__ void k_spmv_seg_fix(int64_t nseg, const int *__restrict__ ai,
+                               const int *__restrict__ tail_row, const A *head_part,
+                               const A *tail_part, V *y, const V *z, A alpha, A beta) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const int r = tail_row[s];
+  if (r < 0) return;
+  A acc = tail_part[s];
+  const int64_t end = ai[r + 1];
+  for (int64_t t = s + 1; t < nseg; ++t) {
+    acc += head_part[t];
+    if (end <= (t + 1) * kSegNnz) break;
+  }
+  A o = alpha * acc;
+  if (beta != A(0)) o += beta * A(z[r]);
+  y[r] = V(o);
+}
+
+// ------------------------------------------------------------------ grad
+__global__ void k_value_grad(int n, const int *__restrict__ ai, const int *__restrict__ aj,
+                             const double *__restrict__ lam, const double *__restrict__ x,
+                             double *g) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double l = -lam[r];
+  for (int q = ai[r]; q < ai[r + 1]; ++q) g[q] = l * x[aj[q]];
+}
+
+template <typename T, typename V, typename A>
+void launch_row(Mat &M, const T *av, const V *x, V *y, const V *z, A alpha, A beta,
+                hipStream_t s) {
+  const int n = M.rows;
+  int G = 1;
+  while (G < 64 && G < M.avg_row) G <<= 1;
+  const int64_t threads = static_cast<int64_t>(n) * G;
+  const unsigned grid = static_cast<unsigned>((threads + kBlock - 1) / kBlock);
+  if (grid == 0) return;
+  const int *ai = M.ai.as<int>(), *aj = M.aj.as<int>();
+#define LA_ROW(GG) \
+  hipLaunchKernelGGL((k_spmv_row<GG, T, V, A>), dim3(grid), dim3(kBlock), 0, s, n, ai, aj, av, x, y, z, alpha, beta)
+  switch (G) {
+    case 1: LA_ROW(1); break;
+    case 2: LA_ROW(2); break;
+    case 4: LA_ROW(4); break;
+    case 8: LA_ROW(8); break;
+    case 16: LA_ROW(16); break;
+    case 32: LA_ROW(32); break;
+    default: LA_ROW(64); break;
+  }
+#undef LA_ROW
+  LA_HIP(hipGetLastError());
+}
+
+template <typename T, typename V, typename A>
+void launch_seg(Mat &M, const T *av, const V *x, V *y, const V *z, A alpha, A beta,
+                hipStream_t s) {
+  if (!M.seg_ready) spmv_prepare_segments(M);
+  const int64_t nseg = M.nseg;
+  const unsigned grid = static_cast<unsigned>((nseg + 3) / 4);
+  A *head = static_cast<A *>(M.seg_head.p), *tail = static_cast<A *>(M.seg_tail.p);
+  hipLaunchKernelGGL((k_spmv_seg<T, V, A>), dim3(grid), dim3(kBlock), 0, s, M.rows, M.nnz,
+                     nseg, M.ai.as<int>(), M.aj.as<int>(), av, x, M.seg_first_row.as<int>(),
+                     head, tail, M.seg_tail_row.as<int>(), y, z, alpha, beta);
+  LA_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_spmv_seg_fix<V, A>), dim3((nseg + 255) / 256), dim3(256), 0, s, nseg,
+                     M.ai.as<int>(), M.seg_tail_row.as<int>(), head, tail, y, z, alpha, beta);
+  LA_HIP(hipGetLastError());
+}
+
+template <typename T, typename V, typename A>
+void dispatch_alg(Mat &M, la_spmv_alg alg, const void *av, const void *x, void *y,
+                  const void *z, double alpha, double beta, hipStream_t s) {
+  if (alg == LA_SPMV_AUTO)
+    alg = (M.max_row > 8 * std::max(1.0, M.avg_row) && M.max_row > 64) ? LA_SPMV_ALBUS
+                                                                         : LA_SPMV_ROW;
+  const T *a = static_cast<const T *>(av);
+  const V *xx = static_cast<const V *>(x);
+  V *yy = static_cast<V *>(y);
+  const V *zz = static_cast<const V *>(z);
+  if (alg == LA_SPMV_ROW)
+    launch_row<T, V, A>(M, a, xx, yy, zz, A(alpha), A(beta), s);
+  else
+    launch_seg<T, V, A>(M, a, xx, yy, zz, A(alpha), A(beta), s);
+}
+}  // namespace
+
+std::unique_ptr<Mat> make_mat(int rows, int cols, int base, const int *ai, const int *aj,
+                              const double *av, hipStream_t s) {
+  LA_CHECK_ARG(rows >= 0 && cols >= 0, "negative dimension");
+  LA_CHECK_ARG(ai != nullptr, "ai is null");
+  auto M = std::make_unique<Mat>();
+  M->rows = rows;
+  M->cols = cols;
+  M->stream = s;
+  const int64_t nnz = static_cast<int64_t>(ai[rows]) - ai[0];
+  LA_CHECK_ARG(ai[0] == base, "ai[0] must equal base");
+  LA_CHECK_ARG(nnz >= 0 && nnz < INT32_MAX, "nnz out of int32 range");
+  LA_CHECK_ARG(nnz == 0 || (aj && av), "aj/av null");
+  M->nnz = nnz;
+  M->h_ai.resize(static_cast<size_t>(rows) + 1);
+  M->h_aj.resize(static_cast<size_t>(nnz));
+  int max_row = 0;
+  for (int i = 0; i <= rows; ++i) M->h_ai[i] = ai[i] - base;
+  for (int i = 0; i < rows; ++i) {
+    const int len = M->h_ai[i + 1] - M->h_ai[i];
+    LA_CHECK_ARG(len >= 0, "row pointers must be non-decreasing");
+    max_row = std::max(max_row, len);
+  }
+  for (int64_t k = 0; k < nnz; ++k) {
+    const int c = aj[k] - base;
+    LA_CHECK_ARG(c >= 0 && c < cols, "column index out of range");
+    M->h_aj[k] = c;
+  }
+  M->max_row = max_row;
+  M->avg_row = rows ? static_cast<double>(nnz) / rows : 0.0;
+  M->ai.alloc(sizeof(int) * (static_cast<size_t>(rows) + 1));
+  M->aj.alloc(sizeof(int) * static_cast<size_t>(nnz));
+  M->ai.upload(M->h_ai.data(), sizeof(int) * (rows + 1), s);
+  M->aj.upload(M->h_aj.data(), sizeof(int) * nnz, s);
+  M->vals.set_from_host(av, nnz, s);
+  LA_HIP(hipStreamSynchronize(s));
+  return M;
+}
+
+void spmv_prepare_segments(Mat &A) {
+  const int64_t nseg = std::max<int64_t>(1, (A.nnz + kSegNnz - 1) / kSegNnz);
+  A.nseg = nseg;
+  A.seg_first_row.alloc(sizeof(int) * (nseg + 1));
+  A.seg_head.alloc(sizeof(double) * nseg);
+  A.seg_tail.alloc(sizeof(double) * nseg);
+  A.seg_tail_row.alloc(sizeof(int) * nseg);
+  hipLaunchKernelGGL(k_seg_table, dim3((nseg + 1 + 255) / 256), dim3(256), 0, A.stream, A.rows,
+                     nseg, A.ai.as<int>(), A.seg_first_row.as<int>());
+  LA_HIP(hipGetLastError());
+  LA_HIP(hipStreamSynchronize(A.stream));
+  A.seg_ready = true;
+}
+
+void spmv(Mat &A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, double alpha,
+          const void *x, double beta, const void *z, void *y, hipStream_t s) {
+  LA_CHECK_ARG(val >= LA_F64 && val <= LA_BF16, "bad value dtype");
+  LA_CHECK_ARG(vec == LA_F64 || vec == LA_F32, "vectors must be FP64 or FP32");
+  LA_CHECK_ARG(acc == LA_F64 || acc == LA_F32, "accumulation must be FP64 or FP32");
+  LA_CHECK_ARG(alg >= LA_SPMV_ROW && alg <= LA_SPMV_AUTO, "bad SpMV algorithm");
+  LA_CHECK_ARG(A.rows == 0 || (x && y), "null vector");
+  if (A.rows == 0) return;
+  if (!A.vals.has[val]) A.vals.add(val, s);
+  const void *av = A.vals.get(val);
+  if (acc == LA_F64) {
+    LA_CHECK_ARG(val == LA_F64 && vec == LA_F64,
+                 "FP64 accumulation needs FP64 values and vectors");
+    dispatch_alg<double, double, double>(A, alg, av, x, y, z, alpha, beta, s);
+    return;
+  }
+  // FP32 accumulation
+  if (vec == LA_F64) {
+    if (val == LA_F64) dispatch_alg<double, double, float>(A, alg, av, x, y, z, alpha, beta, s);
+    else if (val == LA_F32) dispatch_alg<float, double, float>(A, alg, av, x, y, z, alpha, beta, s);
+    else dispatch_alg<bf16, double, float>(A, alg, av, x, y, z, alpha, beta, s);
+  } else {
+    if (val == LA_F64) dispatch_alg<double, float, float>(A, alg, av, x, y, z, alpha, beta, s);
+    else if (val == LA_F32) dispatch_alg<float, float, float>(A, alg, av, x, y, z, alpha, beta, s);
+    else dispatch_alg<bf16, float, float>(A, alg, av, x, y, z, alpha, beta, s);
+  }
+}
+
+void spmv_transpose(Mat &A, la_dtype val, la_dtype vec, double alpha, const void *x,
+                    double beta, void *y, hipStream_t s) {
+  if (!A.transposed) {
+    // download the FP64 values, transpose on the host (stable), upload
+    std::vector<double> hv(static_cast<size_t>(A.nnz));
+    if (A.nnz)
+      LA_HIP(hipMemcpyAsync(hv.data(), A.vals.get(LA_F64), sizeof(double) * A.nnz,
+                            hipMemcpyDeviceToHost, s));
+    LA_HIP(hipStreamSynchronize(s));
+    std::vector<int> ti(static_cast<size_t>(A.cols) + 1), tj(static_cast<size_t>(A.nnz));
+    std::vector<double> tv(static_cast<size_t>(A.nnz));
+    transpose(A.rows, A.cols, 0, A.h_ai.data(), A.h_aj.data(), hv.data(), ti.data(), tj.data(),
+              tv.data());
+    A.transposed = make_mat(A.cols, A.rows, 0, ti.data(), tj.data(), tv.data(), s);
+  }
+  Mat &T = *A.transposed;
+  const la_dtype acc = (val == LA_F64 && vec == LA_F64) ? LA_F64 : LA_F32;
+  spmv(T, LA_SPMV_AUTO, val, vec, acc, alpha, x, beta, y, y, s);
+}
+
+void csr_value_grad(Mat &A, const double *lambda, const double *x, double *g, hipStream_t s) {
+  if (A.rows == 0) return;
+  hipLaunchKernelGGL(k_value_grad, dim3((A.rows + 255) / 256), dim3(256), 0, s, A.rows,
+                     A.ai.as<int>(), A.aj.as<int>(), lambda, x, g);
+  LA_HIP(hipGetLastError());
+}
+
+}  // namespace laamd

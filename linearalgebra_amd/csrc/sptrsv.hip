@@ -1,0 +1,432 @@
+// Sparse triangular solves on gfx950 and the ILU(k)/IC(k) apply.
+//
+// Analysis (host): level sets by TopologicalSort2
+// (sparse_mat_op/matrix_utils.hpp:637-675); the triangle is stored with its
+// rows in level order (the reference's permuteRow in
+// OptimizedTriangularSolve::analysis, triangle_solve.tpp:87-91) so each
+// level is a contiguous, coalesced stream; columns keep original ids.
+//
+// Solve, two modes:
+//  * LEVEL: one launch per level (the Barrier mode, triangle_solve.tpp:
+//    127-155), captured once into a HIP graph and replayed.
+//  * SYNCFREE: one launch; wavefronts claim chunks of rows in level order
+//    from an atomic ticket (the NoBarrier mode, triangle_solve.tpp:157-187).
+//    x is pre-filled with a sentinel; a row waits until each x_j it reads is
+//    no longer the sentinel (agent-scope relaxed atomic load, value and
+//    readiness in one word) and publishes its own x_i the same way. Chunks
+//    never span levels, so rows inside one wavefront never wait on each
+//    other, and every wait is on an earlier ticket, which a resident
+//    wavefront already holds: the solve always drains. Waits are bounded;
+//    a timeout sets an error flag instead of hanging.
+// Row formula (triangle_solve.hpp:32-71): x_i = (b_i - sum_j T_ij x_j) / d_i.
+#include <algorithm>
+
+#include "dev_util.hpp"
+#include "device.hpp"
+#include "host_algos.hpp"
+
+namespace laamd {
+using namespace dev;
+
+namespace {
+constexpr int kB = 256;
+constexpr unsigned kSpinLimit = 1u << 22;
+
+template <typename T>
+__device__ __forceinline__ auto ldt(const T *p, int64_t i) {
+  return ldm(p, i);
+}
+
+template <int G, typename T, typename V, typename A, bool DIAG>
+__global__ __launch_bounds__(kB) void k_trsv_level(int k0, int k1, const int *__restrict__ rowptr,
+                                                   const int *__restrict__ col,
+                                                   const T *__restrict__ val,
+                                                   const T *__restrict__ diag,
+                                                   const int *__restrict__ perm,
+                                                   const V *__restrict__ b, V *x) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  const int64_t k = k0 + t / G;
+  const int sub = threadIdx.x & (G - 1);
+  if (k >= k1) return;
+  const int rs = rowptr[k], re = rowptr[k + 1];
+  A acc = 0;
+  for (int q = rs + sub; q < re; q += G) acc += A(ldt(val, q)) * A(x[ldi(col, q)]);
+  acc = group_sum<G>(acc);
+  if (sub == 0) {
+    const int r = perm[k];
+    A v = A(b[r]) - acc;
+    if constexpr (DIAG) v = v / A(ldt(diag, k));
+    x[r] = V(v);
+  }
+}
+
+template <typename V>
+__global__ void k_fill_sentinel(int64_t n, V *x, unsigned *counter) {
+  using U = typename Sentinel<V>::U;
+  U *xb = reinterpret_cast<U *>(x);
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB)
+    xb[t] = Sentinel<V>::bits;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
+}
+
+template <int G, typename T, typename V, typename A, bool DIAG>
+__global__ __launch_bounds__(kB) void k_trsv_syncfree(
+    int nchunks, const int *__restrict__ chunk_start, const int *__restrict__ rowptr,
+    const int *__restrict__ col, const T *__restrict__ val, const T *__restrict__ diag,
+    const int *__restrict__ perm, const V *__restrict__ b, V *x, unsigned *counter,
+    int *err) {
+  const int lane = threadIdx.x & 63;
+  const int gi = lane / G, sub = lane & (G - 1);
+  for (;;) {
+    int c = 0;
+    if (lane == 0) c = static_cast<int>(atomicAdd(counter, 1u));
+    c = __shfl(c, 0, 64);
+    if (c >= nchunks) break;
+    const int k0 = chunk_start[c], k1 = chunk_start[c + 1];
+    const int k = k0 + gi;
+    if (k < k1) {
+      const int rs = rowptr[k], re = rowptr[k + 1];
+      A acc = 0;
+      bool timed_out = false;
+      for (int q = rs + sub; q < re; q += G) {
+        const int j = ldi(col, q);
+        const A a = A(ldt(val, q));
+        V xv = load_ready(x + j);
+        unsigned spins = 0;
+        while (is_sentinel(xv)) {
+          __builtin_amdgcn_s_sleep(1);
+          xv = load_ready(x + j);
+          if (++spins > kSpinLimit) {
+            timed_out = true;
+            xv = V(0);
+            break;
+          }
+        }
+        acc += a * A(xv);
+      }
+      if (timed_out) atomicOr(err, 1);
+      acc = group_sum<G>(acc);
+      if (sub == 0) {
+        const int r = perm[k];
+        A v = A(b[r]) - acc;
+        if constexpr (DIAG) v = v / A(ldt(diag, k));
+        publish(x + r, V(v));
+      }
+    }
+  }
+}
+
+inline int pick_G(double avg) {
+  int G = 1;
+  while (G < 64 && G < avg) G <<= 1;
+  return G;
+}
+
+template <typename T, typename V, typename A, bool DIAG>
+void launch_level_range(const TrsvPlan &P, int l0, int l1, const T *val, const T *diag,
+                        const V *b, V *x, hipStream_t s) {
+  for (int l = l0; l < l1; ++l) {
+    const int k0 = P.level_ptr[l], k1 = P.level_ptr[l + 1];
+    if (k1 <= k0) continue;
+    const int64_t threads = static_cast<int64_t>(k1 - k0) * P.G;
+    const unsigned grid = static_cast<unsigned>((threads + kB - 1) / kB);
+#define LA_LVL(GG)                                                                           \
+  hipLaunchKernelGGL((k_trsv_level<GG, T, V, A, DIAG>), dim3(grid), dim3(kB), 0, s, k0, k1, \
+                     P.rowptr.as<int>(), P.col.as<int>(), val, diag, P.perm.as<int>(), b, x)
+    switch (P.G) {
+      case 1: LA_LVL(1); break;
+      case 2: LA_LVL(2); break;
+      case 4: LA_LVL(4); break;
+      case 8: LA_LVL(8); break;
+      case 16: LA_LVL(16); break;
+      case 32: LA_LVL(32); break;
+      default: LA_LVL(64); break;
+    }
+#undef LA_LVL
+  }
+  LA_HIP(hipGetLastError());
+}
+
+template <typename T, typename V, typename A, bool DIAG>
+void launch_syncfree(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x, hipStream_t s) {
+  const int64_t n = P.n;
+  unsigned fg = static_cast<unsigned>(std::min<int64_t>(4096, (n + kB - 1) / kB));
+  if (fg == 0) fg = 1;
+  hipLaunchKernelGGL(k_fill_sentinel<V>, dim3(fg), dim3(kB), 0, s, n, x, P.counter.as<unsigned>());
+  LA_HIP(hipGetLastError());
+  // persistent-style grid: at most what the chip keeps resident
+  const unsigned waves_needed = static_cast<unsigned>(P.nchunks);
+  unsigned grid = std::min<unsigned>((waves_needed + 3) / 4, 256u * 8u);
+  if (grid == 0) grid = 1;
+#define LA_SF(GG)                                                                              \
+  hipLaunchKernelGGL((k_trsv_syncfree<GG, T, V, A, DIAG>), dim3(grid), dim3(kB), 0, s,        \
+                     P.nchunks, P.chunk_start.as<int>(), P.rowptr.as<int>(), P.col.as<int>(), \
+                     val, diag, P.perm.as<int>(), b, x, P.counter.as<unsigned>(),            \
+                     P.err_flag.as<int>())
+  switch (P.G) {
+    case 1: LA_SF(1); break;
+    case 2: LA_SF(2); break;
+    case 4: LA_SF(4); break;
+    case 8: LA_SF(8); break;
+    case 16: LA_SF(16); break;
+    case 32: LA_SF(32); break;
+    default: LA_SF(64); break;
+  }
+#undef LA_SF
+  LA_HIP(hipGetLastError());
+}
+
+template <typename T, typename V, typename A>
+void solve_typed(TrsvPlan &P, la_trsv_mode mode, const void *valp, const void *diagp,
+                 const void *b, void *x, hipStream_t s) {
+  const T *val = static_cast<const T *>(valp);
+  const T *diag = static_cast<const T *>(diagp);
+  const V *bb = static_cast<const V *>(b);
+  V *xx = static_cast<V *>(x);
+  if (mode == LA_TRSV_SYNCFREE) {
+    if (P.unit) launch_syncfree<T, V, A, false>(P, val, diag, bb, xx, s);
+    else launch_syncfree<T, V, A, true>(P, val, diag, bb, xx, s);
+  } else {
+    if (P.unit) launch_level_range<T, V, A, false>(P, 0, P.nlevels, val, diag, bb, xx, s);
+    else launch_level_range<T, V, A, true>(P, 0, P.nlevels, val, diag, bb, xx, s);
+  }
+}
+
+void solve_dispatch(TrsvPlan &P, la_trsv_mode mode, la_dtype val, la_dtype vec, const void *b,
+                    void *x, hipStream_t s) {
+  const void *v = P.nnz ? P.val.get(val) : nullptr;
+  const void *d = P.unit ? nullptr : P.diag.get(val);
+  if (val == LA_F64 && vec == LA_F64) return solve_typed<double, double, double>(P, mode, v, d, b, x, s);
+  if (vec == LA_F64) {
+    if (val == LA_F64) return solve_typed<double, double, float>(P, mode, v, d, b, x, s);
+    if (val == LA_F32) return solve_typed<float, double, float>(P, mode, v, d, b, x, s);
+    return solve_typed<bf16, double, float>(P, mode, v, d, b, x, s);
+  }
+  if (val == LA_F64) return solve_typed<double, float, float>(P, mode, v, d, b, x, s);
+  if (val == LA_F32) return solve_typed<float, float, float>(P, mode, v, d, b, x, s);
+  return solve_typed<bf16, float, float>(P, mode, v, d, b, x, s);
+}
+
+hipStream_t capture_stream() {
+  static thread_local hipStream_t cs = nullptr;
+  if (!cs) LA_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  return cs;
+}
+}  // namespace
+
+TrsvPlan::~TrsvPlan() {
+  for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+}
+
+std::unique_ptr<TrsvPlan> make_trsv(int n, int base, const int *ai, const int *aj,
+                                    const double *av, const double *diag, bool upper,
+                                    hipStream_t s) {
+  LA_CHECK_ARG(n >= 0 && ai, "bad triangle");
+  auto P = std::make_unique<TrsvPlan>();
+  P->n = n;
+  P->upper = upper;
+  P->unit = diag == nullptr;
+  P->stream = s;
+  const int64_t nnz = static_cast<int64_t>(ai[n]) - ai[0];
+  LA_CHECK_ARG(ai[0] == base, "ai[0] must equal base");
+  LA_CHECK_ARG(nnz == 0 || (aj && av), "aj/av null");
+  P->nnz = nnz;
+  P->h_ai.resize(static_cast<size_t>(n) + 1);
+  P->h_aj.resize(static_cast<size_t>(nnz));
+  P->h_av.assign(av, av + nnz);
+  for (int i = 0; i <= n; ++i) P->h_ai[i] = ai[i] - base;
+  for (int i = 0; i < n; ++i)
+    for (int q = P->h_ai[i]; q < P->h_ai[i + 1]; ++q) {
+      const int c = aj[q] - base;
+      LA_CHECK_ARG(c >= 0 && c < n, "column out of range");
+      LA_CHECK_ARG(upper ? c > i : c < i,
+                   "triangular solve needs a strictly triangular matrix (diagonal apart)");
+      P->h_aj[q] = c;
+    }
+  if (diag) P->h_diag.assign(diag, diag + n);
+  // level sets
+  std::vector<int> iperm(std::max(n, 1)), prefix(static_cast<size_t>(n) + 2);
+  const int L = topo_sort2(n, 0, P->h_ai.data(), P->h_aj.data(), upper, iperm.data(), prefix.data());
+  P->nlevels = L;
+  P->level_ptr.assign(prefix.begin(), prefix.begin() + L + 1);
+  // level-ordered copy
+  std::vector<int> rp(static_cast<size_t>(n) + 1), cl(static_cast<size_t>(nnz));
+  std::vector<double> vl(static_cast<size_t>(nnz)), dg(diag ? n : 0);
+  rp[0] = 0;
+  for (int k = 0; k < n; ++k) {
+    const int r = iperm[k];
+    const int len = P->h_ai[r + 1] - P->h_ai[r];
+    std::copy(P->h_aj.begin() + P->h_ai[r], P->h_aj.begin() + P->h_ai[r + 1], cl.begin() + rp[k]);
+    std::copy(P->h_av.begin() + P->h_ai[r], P->h_av.begin() + P->h_ai[r + 1], vl.begin() + rp[k]);
+    rp[k + 1] = rp[k] + len;
+    if (diag) dg[k] = diag[r];
+  }
+  P->G = pick_G(n ? static_cast<double>(nnz) / n : 1.0);
+  // chunks: 64/G rows, never spanning a level
+  const int R = 64 / P->G;
+  std::vector<int> chunks;
+  for (int l = 0; l < L; ++l)
+    for (int k = P->level_ptr[l]; k < P->level_ptr[l + 1]; k += R) chunks.push_back(k);
+  chunks.push_back(n);
+  P->nchunks = static_cast<int>(chunks.size()) - 1;
+  P->rowptr.alloc(sizeof(int) * (static_cast<size_t>(n) + 1));
+  P->col.alloc(sizeof(int) * static_cast<size_t>(nnz));
+  P->perm.alloc(sizeof(int) * static_cast<size_t>(std::max(n, 1)));
+  P->chunk_start.alloc(sizeof(int) * chunks.size());
+  P->counter.alloc(sizeof(unsigned) * 4);
+  P->err_flag.alloc(sizeof(int) * 4);
+  P->rowptr.upload(rp.data(), sizeof(int) * (n + 1), s);
+  P->col.upload(cl.data(), sizeof(int) * nnz, s);
+  P->perm.upload(iperm.data(), sizeof(int) * n, s);
+  P->chunk_start.upload(chunks.data(), sizeof(int) * chunks.size(), s);
+  LA_HIP(hipMemsetAsync(P->err_flag.p, 0, sizeof(int) * 4, s));
+  LA_HIP(hipMemsetAsync(P->counter.p, 0, sizeof(unsigned) * 4, s));
+  P->val.set_from_host(vl.data(), nnz, s);
+  if (diag) P->diag.set_from_host(dg.data(), n, s);
+  LA_HIP(hipStreamSynchronize(s));
+  return P;
+}
+
+void trsv_add_dtype(TrsvPlan &T, la_dtype dt) {
+  if (T.nnz) T.val.add(dt, T.stream);
+  else if (!T.val.has[LA_F64]) { /* empty triangle: nothing stored */ }
+  if (!T.unit) T.diag.add(dt, T.stream);
+  if (T.transposed) trsv_add_dtype(*T.transposed, dt);
+  LA_HIP(hipStreamSynchronize(T.stream));
+}
+
+void trsv_solve(TrsvPlan &T, la_trsv_mode mode, la_dtype val, la_dtype vec, const void *b,
+                void *x, hipStream_t s) {
+  LA_CHECK_ARG(mode == LA_TRSV_LEVEL || mode == LA_TRSV_SYNCFREE, "bad trsv mode");
+  LA_CHECK_ARG(val >= LA_F64 && val <= LA_BF16, "bad value dtype");
+  LA_CHECK_ARG(vec == LA_F64 || vec == LA_F32, "vectors must be FP64 or FP32");
+  LA_CHECK_ARG(T.n == 0 || (b && x), "null vector");
+  LA_CHECK_ARG(b != x || T.n == 0, "b and x must be distinct");
+  if (T.n == 0) return;
+  if ((T.nnz && !T.val.has[val]) || (!T.unit && !T.diag.has[val])) trsv_add_dtype(T, val);
+  if (mode == LA_TRSV_SYNCFREE || T.nlevels <= 2) {
+    solve_dispatch(T, mode, val, vec, b, x, s);
+    return;
+  }
+  // LEVEL: replay a captured graph of the per-level launches
+  auto key = std::make_tuple(static_cast<int>(val), static_cast<int>(vec), b, x);
+  auto it = T.graphs.find(key);
+  if (it == T.graphs.end()) {
+    if (T.graphs.size() >= 16) {
+      for (auto &kv : T.graphs) (void)hipGraphExecDestroy(kv.second);
+      T.graphs.clear();
+    }
+    hipStream_t cs = capture_stream();
+    hipGraph_t g = nullptr;
+    LA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    try {
+      solve_dispatch(T, mode, val, vec, b, x, cs);
+    } catch (...) {
+      (void)hipStreamEndCapture(cs, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    LA_HIP(hipStreamEndCapture(cs, &g));
+    hipGraphExec_t ge = nullptr;
+    LA_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    it = T.graphs.emplace(key, ge).first;
+  }
+  LA_HIP(hipGraphLaunch(it->second, s));
+}
+
+TrsvPlan &trsv_transposed(TrsvPlan &T) {
+  if (!T.transposed) {
+    std::vector<int> ti(static_cast<size_t>(T.n) + 1), tj(static_cast<size_t>(T.nnz));
+    std::vector<double> tv(static_cast<size_t>(T.nnz));
+    transpose(T.n, T.n, 0, T.h_ai.data(), T.h_aj.data(), T.h_av.data(), ti.data(), tj.data(),
+              tv.data());
+    T.transposed = make_trsv(T.n, 0, ti.data(), tj.data(), tv.data(),
+                             T.unit ? nullptr : T.h_diag.data(), !T.upper, T.stream);
+    for (int dt = 1; dt < 3; ++dt)
+      if (T.val.has[dt] || T.diag.has[dt]) trsv_add_dtype(*T.transposed, static_cast<la_dtype>(dt));
+  }
+  return *T.transposed;
+}
+
+bool trsv_check_timeout(TrsvPlan &T, hipStream_t s) {
+  int flag = 0;
+  LA_HIP(hipMemcpyAsync(&flag, T.err_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  LA_HIP(hipStreamSynchronize(s));
+  if (flag) LA_HIP(hipMemsetAsync(T.err_flag.p, 0, sizeof(int), s));
+  return flag != 0;
+}
+
+// ------------------------------------------------------------------ ILU / IC
+std::unique_ptr<Prec> make_prec_ilu(int n, int base, const int *fai, const int *faj,
+                                    const double *fav, hipStream_t s) {
+  auto P = std::make_unique<Prec>();
+  P->n = n;
+  P->stream = s;
+  std::vector<int> Lai(static_cast<size_t>(n) + 1), Uai(static_cast<size_t>(n) + 1);
+  int64_t ln = 0, un = 0;
+  split_ldu(n, base, fai, faj, fav, Lai.data(), nullptr, nullptr, nullptr, Uai.data(), nullptr,
+            nullptr, &ln, &un);
+  std::vector<int> Laj(ln), Uaj(un);
+  std::vector<double> Lav(ln), Uav(un), D(n);
+  split_ldu(n, base, fai, faj, fav, Lai.data(), Laj.data(), Lav.data(), D.data(), Uai.data(),
+            Uaj.data(), Uav.data(), &ln, &un);
+  for (int i = 0; i < n; ++i)
+    if (D[i] == 0.0) throw la_error(LA_ERR_NUMERIC, "ILU factor has a zero or missing diagonal");
+  P->lower = make_trsv(n, base, Lai.data(), Laj.data(), Lav.data(), nullptr, false, s);
+  P->upper = make_trsv(n, base, Uai.data(), Uaj.data(), Uav.data(), D.data(), true, s);
+  P->tmp[0].alloc(sizeof(double) * std::max(n, 1));
+  P->tmp[1].alloc(sizeof(float) * std::max(n, 1));
+  return P;
+}
+
+std::unique_ptr<Prec> make_prec_ic(int n, int base, const int *rai, const int *raj,
+                                   const double *rav, hipStream_t s) {
+  auto P = std::make_unique<Prec>();
+  P->ic = true;
+  P->n = n;
+  P->stream = s;
+  std::vector<int> Lai(static_cast<size_t>(n) + 1), Uai(static_cast<size_t>(n) + 1);
+  int64_t ln = 0, un = 0;
+  split_ldu(n, base, rai, raj, rav, Lai.data(), nullptr, nullptr, nullptr, Uai.data(), nullptr,
+            nullptr, &ln, &un);
+  LA_CHECK_ARG(ln == 0, "IC factor must be upper triangular");
+  std::vector<int> Uaj(un);
+  std::vector<double> Uav(un), D(n);
+  split_ldu(n, base, rai, raj, rav, Lai.data(), nullptr, nullptr, D.data(), Uai.data(),
+            Uaj.data(), Uav.data(), &ln, &un);
+  for (int i = 0; i < n; ++i)
+    if (!(D[i] > 0.0)) throw la_error(LA_ERR_NUMERIC, "IC factor has a non-positive diagonal");
+  std::vector<int> ti(static_cast<size_t>(n) + 1), tj(un);
+  std::vector<double> tv(un);
+  transpose(n, n, base, Uai.data(), Uaj.data(), Uav.data(), ti.data(), tj.data(), tv.data());
+  P->lower = make_trsv(n, base, ti.data(), tj.data(), tv.data(), D.data(), false, s);
+  P->upper = make_trsv(n, base, Uai.data(), Uaj.data(), Uav.data(), D.data(), true, s);
+  P->tmp[0].alloc(sizeof(double) * std::max(n, 1));
+  P->tmp[1].alloc(sizeof(float) * std::max(n, 1));
+  return P;
+}
+
+void prec_add_dtype(Prec &P, la_dtype dt) {
+  trsv_add_dtype(*P.lower, dt);
+  trsv_add_dtype(*P.upper, dt);
+}
+
+void prec_apply(Prec &P, la_trsv_mode mode, la_dtype val, la_dtype vec, const void *r, void *z,
+                hipStream_t s) {
+  void *t = P.tmp[vec == LA_F64 ? 0 : 1].p;
+  trsv_solve(*P.lower, mode, val, vec, r, t, s);
+  trsv_solve(*P.upper, mode, val, vec, t, z, s);
+}
+
+void prec_apply_transpose(Prec &P, la_trsv_mode mode, la_dtype val, la_dtype vec,
+                          const void *r, void *z, hipStream_t s) {
+  if (P.ic) return prec_apply(P, mode, val, vec, r, z, s);  // M = R^T R is symmetric
+  void *t = P.tmp[vec == LA_F64 ? 0 : 1].p;
+  // (LU)^-T = L^-T U^-T: solve U^T first (a lower solve), then L^T
+  trsv_solve(trsv_transposed(*P.upper), mode, val, vec, r, t, s);
+  trsv_solve(trsv_transposed(*P.lower), mode, val, vec, t, z, s);
+}
+
+}  // namespace laamd

@@ -1,0 +1,164 @@
+"""GPU SpMV parity (through the C ABI) against the oracle's SerialSPMV
+restatement (sparse_mat_op/spmv.hpp:12-29).
+
+Tolerances: FP64 values/vectors/accumulation -- norm-wise relative error
+<= 1e-12 and element-wise |y - ref| <= 1e-12 * (|alpha| |A| |x| + |beta| |y0|)_i
+(only summation order / FMA rounding may differ). FP32 / BF16 storage is
+checked against the FP64 oracle with the storage rounding bound:
+<= 2e-6 (FP32) and <= 2e-2 (BF16) of (|A| |x|)_i.
+"""
+import numpy as np
+import pytest
+
+import linearalgebra_amd as la
+from helpers import as_lib_csr, from_dense, orc, random_sparse, read_mtx
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    cs = [("ex5", lambda: read_mtx("ex5.mtx")),
+          ("rdist1", lambda: read_mtx("rdist1.mtx")),
+          ("rdist1_base1", lambda: read_mtx("rdist1.mtx", base=1)),
+          ("poisson2d_64", lambda: _o(la.gen.poisson2d(64))),
+          ("laplace3d_16", lambda: _o(la.gen.laplace3d(16))),
+          ("powerlaw_longrows", lambda: _o(la.gen.powerlaw(20000, 16.0, 2.0, 6000, seed=5))),
+          ("aniso27_10", lambda: _o(la.gen.aniso27(10, 9, 8))),
+          ("empty_rows", lambda: from_dense(_empty_rows())),
+          ("one_long_row", lambda: from_dense(_one_long_row())),
+          ("all_empty", lambda: orc.CSR(7, np.zeros(8), [], [])),
+          ("one_by_one", lambda: from_dense([[3.0]]))]
+    return cs
+
+
+def _o(A):
+    return orc.CSR(A.n, A.ai, A.aj, A.av, ncols=A.ncols)
+
+
+def _empty_rows():
+    D = random_sparse(300, 0.05, seed=1)
+    D[::7] = 0.0
+    D[-5:] = 0.0
+    D[:3] = 0.0
+    return D
+
+
+def _one_long_row():
+    D = random_sparse(50, 0.02, seed=2, ncols=3000)
+    D[17] = np.random.default_rng(3).standard_normal(3000)
+    return D
+
+
+def _absbound(A, x):
+    B = orc.CSR(A.n, A.ai, A.aj, np.abs(A.av), ncols=A.ncols)
+    return orc.spmv(B, np.abs(x))
+
+
+@pytest.mark.parametrize("name,make", _cases(), ids=[c[0] for c in _cases()])
+@pytest.mark.parametrize("alg", [la.SPMV_ROW, la.SPMV_SEGSUM, la.SPMV_ALBUS, la.SPMV_AUTO])
+def test_spmv_f64(dev, name, make, alg):
+    import torch
+    A = make()
+    print("case", name, "alg", alg, "n", A.n, "nnz", A.nnz, flush=True)
+    x = la.gen.uniform(A.ncols, 7)
+    y0 = la.gen.uniform(A.n, 8)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    xt = torch.tensor(x, device=dev)
+    for alpha, beta in [(1.0, 0.0), (2.5, -0.5), (-1.0, 1.0)]:
+        yt = torch.tensor(y0, device=dev)
+        dA.spmv(xt, yt, alpha=alpha, beta=beta, alg=alg)
+        ref = orc.spmv(A, x, y0, alpha=alpha, beta=beta)
+        got = yt.cpu().numpy()
+        scale = abs(alpha) * _absbound(A, x) + abs(beta) * np.abs(y0)
+        assert np.all(np.abs(got - ref) <= 1e-12 * scale + 1e-300), (name, alpha, beta)
+        if np.linalg.norm(ref) > 0:
+            assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-12 or \
+                np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(scale)
+
+
+@pytest.mark.parametrize("alg", [la.SPMV_ROW, la.SPMV_ALBUS])
+def test_spmv_beta_zero_ignores_y(dev, alg):
+    """beta == 0 never reads y (SerialSPMV, spmv.hpp:26): NaN in y stays out."""
+    import torch
+    A = _o(la.gen.poisson2d(40))
+    x = la.gen.uniform(A.n, 9)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    y = torch.full((A.n,), float("nan"), dtype=torch.float64, device=dev)
+    dA.spmv(torch.tensor(x, device=dev), y, alg=alg)
+    assert torch.isfinite(y).all()
+    np.testing.assert_allclose(y.cpu().numpy(), orc.spmv(A, x), rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("name,make", _cases()[:7], ids=[c[0] for c in _cases()[:7]])
+@pytest.mark.parametrize("val,bound", [(la.F32, 2e-6), (la.BF16, 2e-2)])
+@pytest.mark.parametrize("alg", [la.SPMV_ROW, la.SPMV_ALBUS])
+def test_spmv_lowprec(dev, name, make, val, bound, alg):
+    import torch
+    A = make()
+    print("case", name, "val", val, "alg", alg, flush=True)
+    x = la.gen.uniform(A.ncols, 7)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    ref = orc.spmv(A, x)
+    scale = _absbound(A, x)
+    for vec in (torch.float32, torch.float64):
+        xt = torch.tensor(x, device=dev, dtype=vec)
+        yt = torch.zeros(A.n, device=dev, dtype=vec)
+        dA.spmv(xt, yt, alg=alg, val=val, acc=la.F32)
+        got = yt.double().cpu().numpy()
+        assert np.all(np.abs(got - ref) <= bound * scale + 1e-30), (name, vec)
+
+
+def test_spmv_f64_storage_f32_accumulate(dev):
+    import torch
+    A = _o(la.gen.powerlaw(5000, 16.0, 1.0, 20000, seed=9))
+    x = la.gen.uniform(A.n, 7)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    yt = torch.zeros(A.n, device=dev, dtype=torch.float64)
+    dA.spmv(torch.tensor(x, device=dev), yt, val=la.F64, acc=la.F32)
+    ref = orc.spmv(A, x)
+    assert np.all(np.abs(yt.cpu().numpy() - ref) <= 2e-6 * _absbound(A, x))
+
+
+@pytest.mark.parametrize("name,make", _cases()[:3] + _cases()[6:8],
+                         ids=[c[0] for c in _cases()[:3] + _cases()[6:8]])
+def test_spmv_transpose(dev, name, make):
+    import torch
+    A = make()
+    At = orc.transpose(A)
+    x = la.gen.uniform(A.n, 3)
+    y0 = la.gen.uniform(A.ncols, 4)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    yt = torch.tensor(y0, device=dev)
+    dA.spmv_transpose(torch.tensor(x, device=dev), yt, alpha=1.5, beta=0.25)
+    ref = orc.spmv(At, x, y0, alpha=1.5, beta=0.25)
+    scale = 1.5 * _absbound(At, x) + 0.25 * np.abs(y0)
+    assert np.all(np.abs(yt.cpu().numpy() - ref) <= 1e-12 * scale + 1e-300)
+
+
+def test_value_grad(dev):
+    import torch
+    A = read_mtx("rdist1.mtx")
+    lam = la.gen.uniform(A.n, 1)
+    x = la.gen.uniform(A.n, 2)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    g = torch.zeros(A.nnz, dtype=torch.float64, device=dev)
+    dA.value_grad(torch.tensor(lam, device=dev), torch.tensor(x, device=dev), g)
+    rows = np.repeat(np.arange(A.n), np.diff(A.ai))
+    ref = -lam[rows] * x[A.aj - A.base]
+    np.testing.assert_array_equal(g.cpu().numpy(), ref)
+
+
+def test_spmv_repeat_deterministic(dev):
+    """nnz-balanced kernel with cross-segment rows is run-to-run bitwise
+    identical (the fix-up adds partials in a fixed order)."""
+    import torch
+    A = _o(la.gen.powerlaw(20000, 16.0, 2.0, 6000, seed=5))
+    x = torch.tensor(la.gen.uniform(A.n, 7), device=dev)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    outs = []
+    for _ in range(5):
+        y = torch.zeros(A.n, dtype=torch.float64, device=dev)
+        dA.spmv(x, y, alg=la.SPMV_ALBUS)
+        outs.append(y.cpu().numpy())
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
