@@ -1,0 +1,276 @@
+"""Benchmark of the sparse hot path on one MI355X.
+
+Default workload (the configuration the metric is quoted on): 3-D 7-point
+Laplacian on a 128^3 grid (n = 2,097,152, nnz = 14,581,760), FP64 CSR with
+int32 indices, RCM-reordered on the host.
+
+  * value: FP64 CSR SpMV y = A x throughput in GB/s of algorithmic bytes
+    B = nnz*(8+4) + (n+1)*4 + n*8 (x, counted once) + n*8 (y), K timed
+    launches. The matrix is held in three copies used in rotation, so a
+    launch never finds its matrix in the 256 MB Infinity Cache: each timed
+    launch streams A from HBM.
+  * pcg: IC(0)-PCG on b ~ U[-1,1) (seeded) to a 1e-10 true relative residual,
+    plain FP64 vs mixed precision (FP32 or BF16 matrix and factors, FP32
+    vectors, FP64 dot products and FP64 iterative refinement).
+  * roofline: the SpMV kernel against the 8 TB/s HBM peak, its average
+    launch duration from HIP events on the stream it runs on.
+  * cpu_baseline: the oracle's OpenMP SpMV (a restatement of the reference's
+    ParallelSPMV) on this host's cores, timed on the same matrix for a
+    bounded ~10 s sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4]
+       [--no-pcg] [--no-cpu] [--pmc-probe]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("SpMV GB/s (%HBM peak); PCG time to 1e-10: mixed FP32/BF16 vs FP64, "
+          "1 GPU")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n, nnz, s_v=8, s_x=8, s_y=8, beta_nonzero=False):
+    return nnz * (s_v + 4) + (n + 1) * 4 + n * s_x + n * s_y * (2 if beta_nonzero else 1)
+
+
+def build_c2(la, grid=128, rcm=True):
+    t0 = time.time()
+    A = la.gen.laplace3d(grid)
+    t_gen = time.time() - t0
+    t0 = time.time()
+    if rcm:
+        ip, p = la.rcm(A)
+        A = la.permute(A, ip, p)
+    t_rcm = time.time() - t0
+    return A, {"gen_s": round(t_gen, 3), "rcm_s": round(t_rcm, 3)}
+
+
+def time_spmv(la, torch, mats, x, y, steps, warmup, alg, val, acc, stream):
+    """Per-launch HIP-event durations (ms) and the wall time of the K
+    launches (ms), rotating over `mats`."""
+    for i in range(warmup):
+        mats[i % len(mats)].spmv(x, y, alg=alg, val=val, acc=acc)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ev[i][0].record(stream)
+        mats[i % len(mats)].spmv(x, y, alg=alg, val=val, acc=acc)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    durs = [a.elapsed_time(b) for a, b in ev]
+    return durs, wall_ms
+
+
+def cpu_baseline(A, seconds=10.0):
+    from oracle import oracle as orc
+    Ao = orc.CSR(A.n, A.ai, A.aj, A.av)
+    x = np.random.default_rng(1).uniform(-1, 1, A.n)
+    threads = orc.max_threads()
+    orc.spmv(Ao, x, alg="parallel", nthreads=threads)  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        orc.spmv(Ao, x, alg="parallel", nthreads=threads)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or reps >= 2000:
+            break
+    per = el / reps
+    B = spmv_bytes(A.n, A.nnz)
+    return {"value": round(B / per / 1e9, 3), "unit": "GB/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d FP64 SpMVs (oracle ParallelSPMV restatement, OpenMP, %d threads) "
+                      "on the same RCM-ordered 128^3 7-point matrix, %.1f s; %.2f ms per "
+                      "SpMV" % (reps, threads, el, per * 1e3),
+            "cpu_model": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_pcg(la, torch, A, dev, stream, trsv_mode, variants):
+    U = la.split_triangle(A, upper=True)
+    t0 = time.time()
+    R, ok = la.icc(U, 0)
+    t_fact = time.time() - t0
+    if not ok:
+        raise RuntimeError("IC(0) failed")
+    t0 = time.time()
+    P = la.Preconditioner(R, "ic")
+    P.set_stream(stream)
+    t_levels = time.time() - t0
+    dA = la.DeviceMatrix(A, stream=stream)
+    b = torch.tensor(la.gen.uniform(A.n, 2024), device=dev)
+    out = {"preconditioner": "IC(0)", "levels": P.levels(), "factor_s": round(t_fact, 3),
+           "analysis_s": round(t_levels, 3), "trsv_mode": ["level", "syncfree"][trsv_mode],
+           "runs": {}}
+    for name, kw in variants:
+        x = torch.zeros(A.n, dtype=torch.float64, device=dev)
+        # one untimed run per variant so level graphs and precision copies
+        # exist; then the timed one
+        la.pcg(dA, P, b, x, rel_tol=1e-10, max_iters=5000, trsv_mode=trsv_mode, **kw)
+        x.zero_()
+        torch.cuda.synchronize()
+        res = la.pcg(dA, P, b, x, rel_tol=1e-10, max_iters=5000, trsv_mode=trsv_mode, **kw)
+        out["runs"][name] = {"seconds": round(res["seconds"], 5), "iters": res["iters"],
+                             "outer": res["outer_iters"],
+                             "rel_residual": float("%.3e" % res["rel_residual"]),
+                             "converged": res["converged"]}
+        log("pcg", name, out["runs"][name])
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--no-pcg", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only a few SpMV launches + a 1 GiB copy (for rocprofv3 --pmc)")
+    ap.add_argument("--trsv", default="level", choices=["level", "syncfree"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU sharding of the "
+              "path is a design note (row-block halo exchange), not a measured mode.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import linearalgebra_amd as la
+    la.lib()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    if args.config != "c2":
+        raise SystemExit("only --config c2 is a bench line; see tests for c1,c3,c4,c5")
+    A, prep = build_c2(la)
+    n, nnz = A.n, A.nnz
+    log("matrix n=%d nnz=%d prep=%s" % (n, nnz, prep))
+    mats = [la.DeviceMatrix(A, stream=stream) for _ in range(3)]
+    x = torch.tensor(la.gen.uniform(n, 77), device=dev)
+    y = torch.zeros(n, dtype=torch.float64, device=dev)
+
+    if args.pmc_probe:
+        for i in range(6):
+            mats[i % 3].spmv(x, y, alg=la.SPMV_ROW)
+        big = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        dst = torch.empty_like(big)
+        dst.copy_(big)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "spmv x6 + copy 1GiB", "spmv_bytes": spmv_bytes(n, nnz)}))
+        return
+
+    # parity check of the timed kernel against the FP64 oracle (cheap)
+    from oracle import oracle as orc
+    mats[0].spmv(x, y, alg=la.SPMV_ROW)
+    ref = orc.spmv(orc.CSR(n, A.ai, A.aj, A.av), x.cpu().numpy())
+    rel = float(np.linalg.norm(y.cpu().numpy() - ref) / np.linalg.norm(ref))
+    if rel > 1e-12:
+        raise SystemExit("SpMV parity failure: rel err %.3e" % rel)
+
+    durs, wall_ms = time_spmv(la, torch, mats, x, y, args.steps, args.warmup, la.SPMV_ROW,
+                              la.F64, la.F64, stream)
+    B = spmv_bytes(n, nnz)
+    avg_ms = float(np.mean(durs))
+    gbs_kernel = B / (avg_ms * 1e-3) / 1e9
+    gbs_wall = B * args.steps / (wall_ms * 1e-3) / 1e9
+    extra = {}
+    # the other SpMV forms on the same matrix, for the record
+    for name, alg, val, acc in [("albus_f64", la.SPMV_ALBUS, la.F64, la.F64)]:
+        d2, _ = time_spmv(la, torch, mats, x, y, 20, 3, alg, val, acc, stream)
+        extra[name] = round(B / (np.mean(d2) * 1e-3) / 1e9, 1)
+    x32 = x.float()
+    y32 = torch.zeros(n, dtype=torch.float32, device=dev)
+    for dt, sv, name in [(la.F32, 4, "f32"), (la.BF16, 2, "bf16")]:
+        for m in mats:
+            m.add_dtype(dt)
+        d2, _ = time_spmv(la, torch, mats, x32, y32, 20, 3, la.SPMV_ROW, dt, la.F32, stream)
+        Bl = spmv_bytes(n, nnz, s_v=sv, s_x=4, s_y=4)
+        extra["row_%s_vals_f32_vecs" % name] = {
+            "GBs": round(Bl / (np.mean(d2) * 1e-3) / 1e9, 1),
+            "us": round(float(np.mean(d2)) * 1e3, 2)}
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_spmv_row_f64_laplace3d_128.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    pcg = None
+    if not args.no_pcg:
+        del mats[1:]
+        mode = la.TRSV_LEVEL if args.trsv == "level" else la.TRSV_SYNCFREE
+        variants = [("fp64", dict(precision=la.PREC_FP64, stop_rule=la.STOP_TRUE_RESIDUAL)),
+                    ("fp64_recurrence", dict(precision=la.PREC_FP64,
+                                             stop_rule=la.STOP_RECURRENCE)),
+                    ("mixed_f32", dict(precision=la.PREC_MIXED_F32, inner_tol=1e-5)),
+                    ("mixed_bf16", dict(precision=la.PREC_MIXED_BF16, inner_tol=1e-4))]
+        pcg = run_pcg(la, torch, A, dev, stream, mode, variants)
+        r = pcg["runs"]
+        best64 = min(r["fp64"]["seconds"], r["fp64_recurrence"]["seconds"])
+        pcg["speedup_mixed_f32_vs_fp64"] = round(best64 / r["mixed_f32"]["seconds"], 3)
+        pcg["speedup_mixed_bf16_vs_fp64"] = round(best64 / r["mixed_bf16"]["seconds"], 3)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(A, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": round(gbs_kernel, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "3-D 7-point Laplacian 128^3, FP64 CSR int32, RCM-ordered; "
+                               "y = A x (row kernel, 8 lanes/row)",
+                   "n": n, "nnz": nnz, "bytes_per_spmv": B,
+                   "pct_hbm_peak": round(100 * gbs_kernel / HBM_PEAK_GBS, 1),
+                   "wall_GBs": round(gbs_wall, 1), "host_prep": prep,
+                   "other_spmv": extra},
+        "roofline": {"bound": "hbm", "achieved": round(gbs_kernel, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs_kernel / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "k_spmv_row<8,double,double,double>",
+                     "avg_launch_us": round(avg_ms * 1e3, 3)},
+        "cpu_baseline": cpu,
+        "pcg": pcg,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

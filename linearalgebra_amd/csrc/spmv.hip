@@ -67,16 +67,32 @@ __global__ void k_seg_table(int n, int64_t nseg, const int *__restrict__ ai, int
   first_row[s] = lo;
 }
 
-// rows [rb, rb+R) owned by this segment, G lanes each, sums read from LDS
+// Rows [rb, rb+R) owned by a segment, G lanes each, sums read from LDS. Row
+// bounds of the first 128 owned rows come from registers preloaded before
+// the product loop (pa = ai[f0+lane], pb = ai[f0+64+lane]); later chunks load
+// them. Every lane executes the shuffles; groups past R drop out after.
 template <int G, typename V, typename A>
-__device__ __forceinline__ void seg_rows(int lane, int rb, int R, int64_t g0, int64_t g1,
+__device__ __forceinline__ void seg_rows(int lane, int chunk, int rb, int R, int64_t g0,
+                                         int64_t g1, int pa, int pb,
                                          const int *__restrict__ ai, const A *p, V *y,
                                          const V *z, A alpha, A beta, A *tail_part,
                                          int64_t s) {
   const int gi = lane / G, sub = lane & (G - 1);
+  int rs = 0, re = 0;
+  if (chunk == 0) {
+    rs = __shfl(pa, gi, 64);
+    const int nx = __shfl(pa, (gi + 1) & 63, 64), nb = __shfl(pb, 0, 64);
+    re = gi + 1 < 64 ? nx : nb;
+  } else if (chunk == 1) {
+    rs = __shfl(pb, gi, 64);
+    const int nx = __shfl(pb, (gi + 1) & 63, 64);
+    re = gi + 1 < 64 ? nx : (gi < R ? ai[rb + 64] : 0);
+  } else if (gi < R) {
+    rs = ai[rb + gi];
+    re = ai[rb + gi + 1];
+  }
   if (gi >= R) return;  // whole groups only
   const int r = rb + gi;
-  const int64_t rs = ai[r], re = ai[r + 1];
   const int64_t qe = re < g1 ? re : g1;
   A acc = 0;
   for (int64_t q = rs + sub; q < qe; q += G) acc += p[q - g0];
@@ -103,6 +119,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv_seg(
   const bool live = s < nseg;
   const int64_t g0 = s * kSegNnz;
   const int64_t g1 = live ? (g0 + kSegNnz < nnz ? g0 + kSegNnz : nnz) : g0;
+  // every load that does not depend on the products goes out first
+  int f0 = 0, f1 = 0, pa = 0, pb = 0;
+  if (live) {
+    f0 = first_row[s];
+    f1 = first_row[s + 1];
+    if (f0 + lane <= f1) pa = ai[f0 + lane];
+    if (f0 + 64 + lane <= f1) pb = ai[f0 + 64 + lane];
+  }
   A *p = sp[wave];
 #pragma unroll
   for (int it = 0; it < kSegE; ++it) {
@@ -113,31 +137,32 @@ __global__ __launch_bounds__(kBlock) void k_spmv_seg(
   }
   __syncthreads();
   if (!live) return;
-  const int f0 = first_row[s], f1 = first_row[s + 1];
   // head: the row that started in an earlier segment (ALBUS bv[2t])
-  if (f0 > 0) {
-    const int64_t hb = ai[f0];
-    if (hb > g0) {
-      const int64_t he = hb < g1 ? hb : g1;
-      A acc = 0;
-      for (int64_t q = g0 + lane; q < he; q += 64) acc += p[q - g0];
-      acc = wave_sum(acc);
-      if (lane == 0) head_part[s] = acc;
-    }
+  const int64_t hb = __shfl(pa, 0, 64);  // ai[f0]
+  if (f0 > 0 && hb > g0) {
+    const int64_t he = hb < g1 ? hb : g1;
+    A acc = 0;
+    for (int64_t q = g0 + lane; q < he; q += 64) acc += p[q - g0];
+    acc = wave_sum(acc);
+    if (lane == 0) head_part[s] = acc;
   }
   if (lane == 0) tail_row[s] = (f1 > f0 && static_cast<int64_t>(ai[f1]) > g1) ? f1 - 1 : -1;
-  for (int rb = f0; rb < f1; rb += 64) {
+  int chunk = 0;
+  for (int rb = f0; rb < f1; rb += 64, ++chunk) {
     const int R = (f1 - rb) < 64 ? (f1 - rb) : 64;
     const int Gs = R > 32 ? 1 : R > 16 ? 2 : R > 8 ? 4 : R > 4 ? 8 : R > 2 ? 16 : R > 1 ? 32 : 64;
+#define LA_SEGROWS(GG) \
+  seg_rows<GG>(lane, chunk, rb, R, g0, g1, pa, pb, ai, p, y, z, alpha, beta, tail_part, s)
     switch (Gs) {
-      case 1: seg_rows<1>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
-      case 2: seg_rows<2>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
-      case 4: seg_rows<4>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
-      case 8: seg_rows<8>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
-      case 16: seg_rows<16>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
-      case 32: seg_rows<32>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
-      default: seg_rows<64>(lane, rb, R, g0, g1, ai, p, y, z, alpha, beta, tail_part, s); break;
+      case 1: LA_SEGROWS(1); break;
+      case 2: LA_SEGROWS(2); break;
+      case 4: LA_SEGROWS(4); break;
+      case 8: LA_SEGROWS(8); break;
+      case 16: LA_SEGROWS(16); break;
+      case 32: LA_SEGROWS(32); break;
+      default: LA_SEGROWS(64); break;
     }
+#undef LA_SEGROWS
   }
 }
 

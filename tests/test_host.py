@@ -1,0 +1,195 @@
+"""Host-side product code (liblaamd.so, CPU entry points of the C ABI)
+against the oracle: level sets, splits, transpose, permutation, RCM and the
+ILU(k)/IC(k) symbolic and numeric phases must be bit-exact; generators are
+checked against closed forms. No GPU is touched."""
+import numpy as np
+import pytest
+
+import linearalgebra_amd as la
+from helpers import as_lib_csr, from_dense, orc, random_sparse, read_mtx, to_dense
+
+
+def _o(A):
+    return orc.CSR(A.n, A.ai, A.aj, A.av, ncols=A.ncols)
+
+
+def _same(a, b):
+    for f in ("ai", "aj", "av"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f))
+
+
+def _matrices():
+    return [("ex5", lambda: read_mtx("ex5.mtx")),
+            ("ex5_base1", lambda: read_mtx("ex5.mtx", base=1)),
+            ("rdist1", lambda: read_mtx("rdist1.mtx")),
+            ("poisson2d_30", lambda: _o(la.gen.poisson2d(30))),
+            ("laplace3d_9", lambda: _o(la.gen.laplace3d(9))),
+            ("rand_dd", lambda: from_dense(random_sparse(150, 0.04, 11, diag=1.0)))]
+
+
+@pytest.mark.parametrize("name,make", _matrices(), ids=[m[0] for m in _matrices()])
+def test_splits_transpose_permute(name, make):
+    A = make()
+    Al = as_lib_csr(A)
+    L1, D1, U1 = orc.split_ldu(A)
+    L2, D2, U2 = la.split_ldu(Al)
+    _same(L1, L2)
+    _same(U1, U2)
+    np.testing.assert_array_equal(D1, D2)
+    for up in (True, False):
+        _same(orc.split_triangle(A, up), la.split_triangle(Al, up))
+    _same(orc.transpose(A), la.transpose(Al))
+    ok1, p1, d1 = orc.diagonal(A)
+    ok2, p2, d2 = la.diagonal(Al)
+    assert ok1 == ok2
+    if ok1:
+        np.testing.assert_array_equal(p1, p2)
+        np.testing.assert_array_equal(d1, d2)
+    rng = np.random.default_rng(1)
+    perm = (rng.permutation(A.n) + A.base).astype(np.int32)
+    iperm = np.empty_like(perm)
+    iperm[perm - A.base] = np.arange(A.n) + A.base
+    _same(orc.permute(A, iperm, perm), la.permute(Al, iperm, perm))
+    _same(orc.permute(A, iperm, None), la.permute(Al, iperm, None))
+
+
+@pytest.mark.parametrize("name,make", _matrices(), ids=[m[0] for m in _matrices()])
+def test_levels(name, make):
+    A = make()
+    L, D, U = orc.split_ldu(A)
+    for T, up in ((L, False), (U, True)):
+        i1, p1, l1 = orc.topo_sort2(T, up)
+        i2, p2, l2 = la.topo_sort2(as_lib_csr(T), up)
+        assert l1 == l2
+        np.testing.assert_array_equal(i1, i2)
+        np.testing.assert_array_equal(p1, p2)
+
+
+@pytest.mark.parametrize("name,make", _matrices(), ids=[m[0] for m in _matrices()])
+def test_rcm_bit_exact(name, make):
+    A = make()
+    i1, p1 = orc.rcm(A)
+    i2, p2 = la.rcm(as_lib_csr(A))
+    np.testing.assert_array_equal(i1, i2)
+    np.testing.assert_array_equal(p1, p2)
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 5])
+@pytest.mark.parametrize("name,make", [m for m in _matrices() if m[0] != "rdist1"],
+                         ids=[m[0] for m in _matrices() if m[0] != "rdist1"])
+def test_iluk_bit_exact(name, make, level):
+    A = make()
+    F1, fu1, ok1 = orc.iluk(A, level)
+    fai, faj, fu2 = la.iluk_symbolic(as_lib_csr(A), level)
+    np.testing.assert_array_equal(F1.ai, fai)
+    np.testing.assert_array_equal(F1.aj, faj)
+    np.testing.assert_array_equal(fu1, fu2)
+    F2 = la.iluk_numeric(as_lib_csr(A), fai, faj, fu2)
+    np.testing.assert_array_equal(F1.av, F2.av)
+
+
+def test_iluk_missing_diagonal_is_an_error():
+    """incomplete_lu.cpp:314-316 prints "no element on diagonal!"; the
+    library returns LA_ERR_NUMERIC."""
+    A = read_mtx("rdist1.mtx")  # only 329 of 4134 diagonals present
+    fai, faj, fu = la.iluk_symbolic(as_lib_csr(A), 0)
+    with pytest.raises(la.LaError) as e:
+        la.iluk_numeric(as_lib_csr(A), fai, faj, fu)
+    assert e.value.code == 5
+
+
+@pytest.mark.parametrize("lvl", [0, 1, 3])
+@pytest.mark.parametrize("name,make", [m for m in _matrices() if m[0] not in ("rdist1", "rand_dd")],
+                         ids=[m[0] for m in _matrices() if m[0] not in ("rdist1", "rand_dd")])
+def test_icc_bit_exact(name, make, lvl):
+    A = make()
+    U = orc.split_triangle(A, True)
+    R1, ok1 = orc.icc(U, lvl)
+    R2, ok2 = la.icc(as_lib_csr(U), lvl)
+    assert ok1 == ok2
+    _same(R1, R2)
+
+
+def test_icc_shift_retry():
+    """incomplete_cholesky_k_2::numeric_factorize doubles a multiplicative
+    shift from 1e-3 until no pivot is negative (incomplete_cholesky.cpp:
+    315-342)."""
+    D = np.array([[1.0, 0.9, 0.9], [0.9, 1.0, 0.9], [0.9, 0.9, 1.0]]) - np.eye(3) * 0.95
+    U = as_lib_csr(orc.split_triangle(from_dense(D), True))
+    R, shift = la.icc_shifted(U, 0)
+    assert shift > 0 and np.all(np.isfinite(R.av))
+    # the same shift sequence on the oracle gives the same factor
+    s, it = 0.0, 0
+    while True:
+        Ro, ok = orc.icc(orc.split_triangle(from_dense(D), True), 0, omega=s)
+        if ok:
+            break
+        s = max(2 * s, 1e-3)
+    assert s == shift
+    np.testing.assert_array_equal(Ro.av, R.av)
+
+
+# ----------------------------------------------------------------- generators
+
+def test_generator_sizes_closed_form():
+    """C1: 256^2 5-point -> nnz 326,656; C2: 128^3 7-point -> 14,581,760;
+    C5: 512^2 edges 523,264 (2 nx ny - nx - ny)."""
+    A = la.gen.poisson2d(256)
+    assert A.n == 65536 and A.nnz == 326656
+    n = 40
+    A3 = la.gen.laplace3d(n)
+    assert A3.nnz == 7 * n ** 3 - 6 * n ** 2
+    assert 2 * 512 * 512 - 1024 == 523264
+
+
+def test_generators_structure():
+    A = la.gen.laplace3d(5, 4, 3)
+    D = to_dense(_o(A))
+    assert np.array_equal(D, D.T) and np.all(np.diag(D) == 6)
+    y = orc.spmv(_o(A), np.ones(A.n))
+    assert np.all(y >= 0) and np.min(y) == 0  # interior rows sum to zero
+    K = la.gen.aniso27(6, 5, 4, aniso=100.0, seed=1)
+    Dk = to_dense(_o(K))
+    assert K.nnz == (3 * 6 - 2) * (3 * 5 - 2) * (3 * 4 - 2)
+    np.testing.assert_allclose(Dk, Dk.T, rtol=1e-15)
+    off = np.abs(Dk).sum(1) - np.abs(np.diag(Dk))
+    assert np.all(np.diag(Dk) >= off - 1e-12)  # weakly diagonally dominant
+    assert np.linalg.eigvalsh(Dk).min() > 0
+    P = la.gen.powerlaw(4000, 16.0, 1.0, 20000, seed=3)
+    lens = np.diff(P.ai)
+    assert 12 < lens.mean() < 20 and lens.min() >= 1
+    Dp = to_dense(_o(P))
+    offp = np.abs(Dp).sum(1) - np.abs(np.diag(Dp))
+    np.testing.assert_allclose(np.diag(Dp), 1.0 + offp, rtol=1e-12)
+    for r in range(P.n):
+        cols = P.aj[P.ai[r]:P.ai[r + 1]]
+        assert np.all(np.diff(cols) > 0)
+    # determinism
+    P2 = la.gen.powerlaw(4000, 16.0, 1.0, 20000, seed=3)
+    np.testing.assert_array_equal(P.aj, P2.aj)
+    np.testing.assert_array_equal(P.av, P2.av)
+
+
+def test_diffusion2d_value_map():
+    nx, ny = 7, 5
+    ne = 2 * nx * ny - nx - ny
+    theta = la.gen.lognormal(ne, 3, 0.0, 0.5)
+    A, ei, ej, vm = la.gen.diffusion2d(nx, ny, theta)
+    D = to_dense(_o(A))
+    np.testing.assert_allclose(D, D.T)
+    for e in range(ne):
+        i, j = ei[e], ej[e]
+        assert A.aj[vm[e, 0]] == j and A.aj[vm[e, 1]] == i
+        assert A.av[vm[e, 0]] == -theta[e] and A.av[vm[e, 1]] == -theta[e]
+        assert A.aj[vm[e, 2]] == i and A.aj[vm[e, 3]] == j
+    # A 1 = number of missing neighbours (grounded boundary)
+    y = orc.spmv(_o(A), np.ones(A.n)).reshape(ny, nx)
+    np.testing.assert_allclose([y[2, 3], y[0, 3], y[0, 0]], [0, 1, 2], atol=1e-14)
+
+
+def test_uniform_lognormal_deterministic():
+    u = la.gen.uniform(100000, 42)
+    assert u.min() >= -1 and u.max() < 1 and abs(u.mean()) < 0.01
+    np.testing.assert_array_equal(u, la.gen.uniform(100000, 42))
+    g = la.gen.lognormal(200000, 7, 0.0, 1.0)
+    assert abs(np.log(g).mean()) < 0.01 and abs(np.log(g).std() - 1) < 0.01
