@@ -64,6 +64,15 @@ typedef enum { LA_LOWER = 0, LA_UPPER = 1 } la_uplo;
 const char *la_last_error(void);
 int la_version(void);
 
+/* Device buffers for callers that hold host arrays (the C++ drop-in layer
+ * in include/laamd/, whose reference signatures take host pointers). No
+ * reference counterpart: the reference has no device. */
+int la_device_alloc(size_t bytes, void **ptr);
+int la_device_free(void *ptr);
+int la_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int la_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int la_device_synchronize(void);
+
 /* ====================== host-side algorithms (no GPU) ================== */
 
 /* matrix_utils.hpp:637-675 TopologicalSort2<L|U>. iperm[n] (base-adjusted),

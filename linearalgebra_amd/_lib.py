@@ -47,6 +47,11 @@ _vp, _i, _d, _i64, _u64 = C.c_void_p, C.c_int, C.c_double, C.c_int64, C.c_uint64
 _SIGS = {
     "la_last_error": [],
     "la_version": [],
+    "la_device_alloc": [C.c_size_t, C.POINTER(_vp)],
+    "la_device_free": [_vp],
+    "la_memcpy_h2d": [_vp, _vp, C.c_size_t],
+    "la_memcpy_d2h": [_vp, _vp, C.c_size_t],
+    "la_device_synchronize": [],
     "la_topo_sort2": [_i, _i, _vp, _vp, _i, _vp, _vp, C.POINTER(_i)],
     "la_split_ldu": [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                      C.POINTER(_i), C.POINTER(_i)],

@@ -108,15 +108,36 @@ __global__ void k_zero(int64_t n, V *x) {
     x[t] = V(0);
 }
 
-// x += alpha p ; r -= alpha q with alpha = sc[0] / sc[1]; partial <r,r>
+// Sum of the nb = kRedBlocks per-block partials of a dot product, in a fixed
+// order, returned to every thread of the block. Every block computes the
+// same value bit for bit, so a consumer kernel needs no separate reduction
+// launch.
+__device__ __forceinline__ double block_sum_partials(const double *__restrict__ part,
+                                                     double *sh) {
+  double v = 0;
+  for (int b = threadIdx.x; b < kRedBlocks; b += kB) v += part[b];
+  v = wave_sum(v);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = 0;
+  for (int w = 0; w < kB / 64; ++w) t += sh[w];
+  return t;
+}
+
+// x += alpha p ; r -= alpha q with alpha = <r,z> / <p,q> from their
+// partials; writes the partials of <r,r>
 template <typename V>
 __global__ __launch_bounds__(kB) void k_cg_update_xr(int64_t n, V *x, V *r, const V *__restrict__ p,
                                                      const V *__restrict__ q,
-                                                     const double *__restrict__ sc,
-                                                     double *part) {
+                                                     const double *__restrict__ part_rz,
+                                                     const double *__restrict__ part_pq,
+                                                     double *part_rr) {
   __shared__ double sh[kB / 64];
-  const double alpha = sc[0] / sc[1];
-  const V a = static_cast<V>(alpha);
+  const double rz = block_sum_partials(part_rz, sh);
+  const double pq = block_sum_partials(part_pq, sh);
+  const V a = static_cast<V>(rz / pq);
   double acc = 0;
   for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
        t += static_cast<int64_t>(gridDim.x) * kB) {
@@ -126,23 +147,22 @@ __global__ __launch_bounds__(kB) void k_cg_update_xr(int64_t n, V *x, V *r, cons
     acc += static_cast<double>(rn) * static_cast<double>(rn);
   }
   const double s = block_reduce<V>(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = s;
 }
 
-// p = z + beta p, beta = sc[3] / sc[0]; thread 0 of block 0 cannot move
-// sc[3] into sc[0] here (other blocks still read sc[0]), the host enqueues
-// that copy after this kernel.
+// p = z + beta p with beta = <r,z>_new / <r,z>_old from their partials
 template <typename V>
-__global__ void k_cg_update_p(int64_t n, V *p, const V *__restrict__ z,
-                              const double *__restrict__ sc) {
-  const double beta = sc[3] / sc[0];
-  const V b = static_cast<V>(beta);
+__global__ __launch_bounds__(kB) void k_cg_update_p(int64_t n, V *p, const V *__restrict__ z,
+                                                    const double *__restrict__ part_new,
+                                                    const double *__restrict__ part_old) {
+  __shared__ double sh[kB / 64];
+  const double rn = block_sum_partials(part_new, sh);
+  const double ro = block_sum_partials(part_old, sh);
+  const V b = static_cast<V>(rn / ro);
   for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
        t += static_cast<int64_t>(gridDim.x) * kB)
     p[t] = z[t] + b * p[t];
 }
-
-__global__ void k_move_scalar(double *sc, int dst, int src) { sc[dst] = sc[src]; }
 
 template <typename V>
 __global__ void k_add_scaled(int64_t n, const V *__restrict__ d, double scale, double *x) {
@@ -240,10 +260,20 @@ void launch_convert_f64(const double *src, int64_t n, la_dtype dst_dt, void *dst
   LA_HIP(hipGetLastError());
 }
 
+namespace {
+void alloc_padded(DevBuf &b, la_dtype dt, int64_t cnt, hipStream_t s) {
+  const int64_t pc = padded_count(cnt);
+  b.alloc(dtype_size(dt) * static_cast<size_t>(pc > 0 ? pc : 1));
+  if (pc > cnt)
+    LA_HIP(hipMemsetAsync(static_cast<char *>(b.p) + dtype_size(dt) * cnt, 0,
+                          dtype_size(dt) * static_cast<size_t>(pc - cnt), s));
+}
+}  // namespace
+
 void MultiValues::set_from_device_f64(const double *d, int64_t cnt, hipStream_t s) {
   count = cnt;
   if (!has[LA_F64]) {
-    v[LA_F64].alloc(sizeof(double) * cnt);
+    alloc_padded(v[LA_F64], LA_F64, cnt, s);
     has[LA_F64] = true;
   }
   if (cnt && d != v[LA_F64].p)
@@ -255,7 +285,7 @@ void MultiValues::set_from_device_f64(const double *d, int64_t cnt, hipStream_t 
 void MultiValues::set_from_host(const double *h, int64_t cnt, hipStream_t s) {
   count = cnt;
   if (!has[LA_F64]) {
-    v[LA_F64].alloc(sizeof(double) * cnt);
+    alloc_padded(v[LA_F64], LA_F64, cnt, s);
     has[LA_F64] = true;
   }
   v[LA_F64].upload(h, sizeof(double) * cnt, s);
@@ -266,7 +296,7 @@ void MultiValues::set_from_host(const double *h, int64_t cnt, hipStream_t s) {
 void MultiValues::add(la_dtype dt, hipStream_t s) {
   if (has[dt]) return;
   if (!has[LA_F64]) throw la_error(LA_ERR_STATE, "no FP64 values to convert from");
-  v[dt].alloc(dtype_size(dt) * static_cast<size_t>(count));
+  alloc_padded(v[dt], dt, count, s);
   launch_convert_f64(v[LA_F64].as<double>(), count, dt, v[dt].p, s);
   has[dt] = true;
 }
@@ -274,7 +304,8 @@ void MultiValues::add(la_dtype dt, hipStream_t s) {
 Scratch::Scratch() {
   partials.alloc(sizeof(double) * kRedBlocks * kMaxDots);
   scalars.alloc(sizeof(double) * 256);
-  LA_HIP(hipHostMalloc(reinterpret_cast<void **>(&host), sizeof(double) * 256, hipHostMallocDefault));
+  LA_HIP(hipHostMalloc(reinterpret_cast<void **>(&host), sizeof(double) * 2 * kRedBlocks,
+                       hipHostMallocDefault));
 }
 Scratch::~Scratch() {
   if (host) (void)hipHostFree(host);
@@ -350,30 +381,46 @@ void fill_zero(int64_t n, la_dtype vec, void *x, hipStream_t s) {
   LA_HIP(hipGetLastError());
 }
 
-void cg_update_xr(int64_t n, la_dtype vec, void *x, void *r, const void *p, const void *q,
-                  const double *sc, double *partials, double *rr_out, hipStream_t s) {
+void dot_partials(int64_t n, la_dtype vec, const void *x, const void *y, double *partials,
+                  hipStream_t s) {
   if (vec == LA_F64)
-    hipLaunchKernelGGL(k_cg_update_xr<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
-                       static_cast<double *>(x), static_cast<double *>(r),
-                       static_cast<const double *>(p), static_cast<const double *>(q), sc, partials);
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                       static_cast<const double *>(x), static_cast<const double *>(y), partials);
   else
-    hipLaunchKernelGGL(k_cg_update_xr<float>, dim3(kRedBlocks), dim3(kB), 0, s, n,
-                       static_cast<float *>(x), static_cast<float *>(r),
-                       static_cast<const float *>(p), static_cast<const float *>(q), sc, partials);
-  LA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kB), 0, s, 1, kRedBlocks, partials, rr_out);
+    hipLaunchKernelGGL(k_dot_partial<float>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                       static_cast<const float *>(x), static_cast<const float *>(y), partials);
   LA_HIP(hipGetLastError());
 }
 
-void cg_update_p(int64_t n, la_dtype vec, void *p, const void *z, double *sc, hipStream_t s) {
+double sum_partials_host(const double *h) {
+  double t = 0;
+  for (int b = 0; b < kRedBlocks; ++b) t += h[b];
+  return t;
+}
+
+void cg_update_xr(int64_t n, la_dtype vec, void *x, void *r, const void *p, const void *q,
+                  const double *part_rz, const double *part_pq, double *part_rr, hipStream_t s) {
   if (vec == LA_F64)
-    hipLaunchKernelGGL(k_cg_update_p<double>, dim3(grid_for(n)), dim3(kB), 0, s, n,
-                       static_cast<double *>(p), static_cast<const double *>(z), sc);
+    hipLaunchKernelGGL(k_cg_update_xr<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                       static_cast<double *>(x), static_cast<double *>(r),
+                       static_cast<const double *>(p), static_cast<const double *>(q), part_rz,
+                       part_pq, part_rr);
   else
-    hipLaunchKernelGGL(k_cg_update_p<float>, dim3(grid_for(n)), dim3(kB), 0, s, n,
-                       static_cast<float *>(p), static_cast<const float *>(z), sc);
+    hipLaunchKernelGGL(k_cg_update_xr<float>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                       static_cast<float *>(x), static_cast<float *>(r),
+                       static_cast<const float *>(p), static_cast<const float *>(q), part_rz,
+                       part_pq, part_rr);
   LA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_move_scalar, dim3(1), dim3(1), 0, s, sc, 0, 3);
+}
+
+void cg_update_p(int64_t n, la_dtype vec, void *p, const void *z, const double *part_new,
+                 const double *part_old, hipStream_t s) {
+  if (vec == LA_F64)
+    hipLaunchKernelGGL(k_cg_update_p<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                       static_cast<double *>(p), static_cast<const double *>(z), part_new, part_old);
+  else
+    hipLaunchKernelGGL(k_cg_update_p<float>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                       static_cast<float *>(p), static_cast<const float *>(z), part_new, part_old);
   LA_HIP(hipGetLastError());
 }
 

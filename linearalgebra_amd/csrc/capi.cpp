@@ -63,6 +63,35 @@ extern "C" {
 const char *la_last_error(void) { return tl_err.c_str(); }
 int la_version(void) { return 100; }
 
+int la_device_alloc(size_t bytes, void **ptr) {
+  LA_TRY
+  LA_CHECK_ARG(ptr, "null output pointer");
+  *ptr = nullptr;
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
+  if (e != hipSuccess) throw la_error(LA_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+  LA_CATCH
+}
+int la_device_free(void *ptr) {
+  LA_TRY
+  if (ptr) LA_HIP(hipFree(ptr));
+  LA_CATCH
+}
+int la_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  LA_TRY
+  if (bytes) LA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  LA_CATCH
+}
+int la_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  LA_TRY
+  if (bytes) LA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  LA_CATCH
+}
+int la_device_synchronize(void) {
+  LA_TRY
+  LA_HIP(hipDeviceSynchronize());
+  LA_CATCH
+}
+
 // ------------------------------------------------------------ host algos
 int la_topo_sort2(int n, int base, const int *ai, const int *aj, int upper, int *iperm,
                   int *prefix, int *nlevels) {

@@ -91,7 +91,14 @@ struct DevBuf {
   }
 };
 
-// Values of a CSR (or factor) kept in up to three precisions.
+// Arrays streamed by the nnz-balanced SpMV are allocated to a whole number
+// of its 512-entry segments and zero-filled past the end, so the kernel's
+// vector loads never need a bounds branch.
+constexpr int64_t kNnzPad = 512;
+inline int64_t padded_count(int64_t n) { return ((n + kNnzPad - 1) / kNnzPad) * kNnzPad; }
+
+// Values of a CSR (or factor) kept in up to three precisions (allocations
+// padded, see padded_count).
 struct MultiValues {
   DevBuf v[3];  // indexed by la_dtype
   bool has[3] = {false, false, false};

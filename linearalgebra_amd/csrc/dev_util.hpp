@@ -91,6 +91,35 @@ __device__ __forceinline__ V from_bits(typename Sentinel<V>::U u) {
   else return __uint_as_float(u);
 }
 
+// ---------------------------------------------------------------- XCD-local
+// MI355X has 8 XCDs, each with its own L2. A load with the sc0 bit skips the
+// CU's L1 and is served by the XCD's L2, which every CU of that XCD writes
+// through to; it therefore sees stores made by other CUs of the same XCD,
+// at L2 latency rather than the memory-side round trip a device-scope load
+// needs. Only valid while producer and consumer run on one XCD.
+__device__ __forceinline__ unsigned long long ld_l2_u64(const void *p) {
+  unsigned long long v;
+  asm volatile("global_load_dwordx2 %0, %1, off sc0\n\ts_waitcnt vmcnt(0)"
+               : "=v"(v)
+               : "v"(p)
+               : "memory");
+  return v;
+}
+__device__ __forceinline__ unsigned ld_l2_u32(const void *p) {
+  unsigned v;
+  asm volatile("global_load_dword %0, %1, off sc0\n\ts_waitcnt vmcnt(0)"
+               : "=v"(v)
+               : "v"(p)
+               : "memory");
+  return v;
+}
+// the XCD this wavefront runs on (HW_REG_XCC_ID, bits 3:0)
+__device__ __forceinline__ unsigned xcc_id() {
+  unsigned v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return v & 0xF;
+}
+
 // Agent-scope relaxed atomics: one location carries both the value and its
 // readiness, so no ordering against other locations is needed.
 template <typename V>

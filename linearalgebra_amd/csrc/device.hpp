@@ -117,11 +117,18 @@ void copy_vec(int64_t n, la_dtype src_dt, const void *src, la_dtype dst_dt, void
               double scale, hipStream_t s);
 void fill_zero(int64_t n, la_dtype vec, void *x, hipStream_t s);
 
-// CG step kernels with device scalars (sc = scalars array)
-//   sc[0] = rz, sc[1] = pq, sc[2] = rr (or true residual), sc[3] = rz_new
+// CG step kernels. Dot products stay as kRedBlocks per-block partials; the
+// consuming kernel sums them itself (same order in every block), the host
+// sums them when it needs the value (sum_partials_host).
+void dot_partials(int64_t n, la_dtype vec, const void *x, const void *y, double *partials,
+                  hipStream_t s);
+double sum_partials_host(const double *h);
+// x += a p, r -= a q, a = <r,z>/<p,q>; partials of <r,r> into part_rr
 void cg_update_xr(int64_t n, la_dtype vec, void *x, void *r, const void *p, const void *q,
-                  const double *sc, double *partials, double *rr_out, hipStream_t s);
-void cg_update_p(int64_t n, la_dtype vec, void *p, const void *z, double *sc, hipStream_t s);
+                  const double *part_rz, const double *part_pq, double *part_rr, hipStream_t s);
+// p = z + (<r,z>_new / <r,z>_old) p
+void cg_update_p(int64_t n, la_dtype vec, void *p, const void *z, const double *part_new,
+                 const double *part_old, hipStream_t s);
 // x64 += scale * d (d in vec precision)
 void add_scaled(int64_t n, la_dtype dvec, const void *d, double scale, double *x64,
                 hipStream_t s);

@@ -86,20 +86,29 @@ CgResult pcg_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_dty
   double *sc = W.sc.scalars.as<double>(), *part = W.sc.partials.as<double>();
   const la_spmv_alg alg = static_cast<la_spmv_alg>(o.spmv_alg);
   CgResult R;
+  // partial-sum slots: <p,q>, <r,r>, <t,t>, and <r,z> double-buffered
+  double *p_pq = part, *p_rr = part + kRedBlocks, *p_tt = part + 2 * kRedBlocks;
+  double *p_rz[2] = {part + 3 * kRedBlocks, part + 4 * kRedBlocks};
+  int cur = 0;
+  (void)sc;
   // r = b - A x ; z = M r ; p = z ; rz = <r,z>
   spmv(A, alg, val, vec, acc, -1.0, x, 1.0, b, r, s);
   apply_M(P, o, val, vec, r, z, n, s);
   copy_vec(n, vec, z, vec, p, 1.0, s);
-  dot_dev(n, vec, r, z, part, sc + 0, s);
+  dot_partials(n, vec, r, z, p_rz[cur], s);
   for (int it = 1; it <= maxiter; ++it) {
     spmv(A, alg, val, vec, acc, 1.0, p, 0.0, q, q, s);
-    dot_dev(n, vec, p, q, part, sc + 1, s);
-    cg_update_xr(n, vec, x, r, p, q, sc, part, sc + 2, s);
+    dot_partials(n, vec, p, q, p_pq, s);
+    cg_update_xr(n, vec, x, r, p, q, p_rz[cur], p_pq, p_rr, s);
+    const double *p_check = p_rr;
     if (true_residual) {
       spmv(A, alg, val, vec, acc, -1.0, x, 1.0, b, t, s);
-      dot_dev(n, vec, t, t, part, sc + 2, s);
+      dot_partials(n, vec, t, t, p_tt, s);
+      p_check = p_tt;
     }
-    const double res = std::sqrt(fetch(W.sc, sc + 2, s));
+    LA_HIP(hipMemcpyAsync(W.sc.host, p_check, sizeof(double) * kRedBlocks, hipMemcpyDeviceToHost, s));
+    LA_HIP(hipStreamSynchronize(s));
+    const double res = std::sqrt(sum_partials_host(W.sc.host));
     R.iters = it;
     R.last_res = res;
     if (o.print_level == 1) std::fprintf(stderr, "%3d = %lg/%lg=%lg\n", it, res, nb, res / nb);
@@ -108,8 +117,9 @@ CgResult pcg_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_dty
       break;
     }
     apply_M(P, o, val, vec, r, z, n, s);
-    dot_dev(n, vec, r, z, part, sc + 3, s);
-    cg_update_p(n, vec, p, z, sc, s);
+    dot_partials(n, vec, r, z, p_rz[cur ^ 1], s);
+    cg_update_p(n, vec, p, z, p_rz[cur ^ 1], p_rz[cur], s);
+    cur ^= 1;
   }
   return R;
 }

@@ -108,33 +108,69 @@ __device__ __forceinline__ void seg_rows(int lane, int chunk, int rb, int R, int
   }
 }
 
+// four consecutive matrix entries / column indices per lane, one vector
+// load each (non-temporal: streamed once per product)
+template <typename T> struct Quad;
+template <> struct Quad<double> { typedef double type __attribute__((ext_vector_type(4))); };
+template <> struct Quad<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct Quad<bf16> { typedef unsigned short type __attribute__((ext_vector_type(4))); };
+typedef int int4v __attribute__((ext_vector_type(4)));
+
+template <typename A>
+__device__ __forceinline__ A widen(double v) { return A(v); }
+template <typename A>
+__device__ __forceinline__ A widen(float v) { return A(v); }
+template <typename A>
+__device__ __forceinline__ A widen(unsigned short v) {
+  return A(__uint_as_float(static_cast<uint32_t>(v) << 16));
+}
+
 template <typename T, typename V, typename A>
 __global__ __launch_bounds__(kBlock) void k_spmv_seg(
     int n, int64_t nnz, int64_t nseg, const int *__restrict__ ai, const int *__restrict__ aj,
     const T *__restrict__ av, const V *__restrict__ x, const int *__restrict__ first_row,
     A *head_part, A *tail_part, int *tail_row, V *y, const V *z, A alpha, A beta) {
+  using Q = typename Quad<T>::type;
+  static_assert(kSegE == 8, "two quads per lane");
   __shared__ A sp[kBlock / 64][kSegNnz];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t s = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + wave;
   const bool live = s < nseg;
   const int64_t g0 = s * kSegNnz;
   const int64_t g1 = live ? (g0 + kSegNnz < nnz ? g0 + kSegNnz : nnz) : g0;
-  // every load that does not depend on the products goes out first
-  int f0 = 0, f1 = 0, pa = 0, pb = 0;
-  if (live) {
-    f0 = first_row[s];
-    f1 = first_row[s + 1];
-    if (f0 + lane <= f1) pa = ai[f0 + lane];
-    if (f0 + 64 + lane <= f1) pb = ai[f0 + 64 + lane];
-  }
   A *p = sp[wave];
+  // lane owns nonzeros g0 + h*256 + 4*lane + e, h in {0,1}, e in {0..3}.
+  // Column and value arrays are padded with zeros to a whole number of
+  // segments, so every load is an unconditional vector load (no branch, no
+  // wait between them); products past the segment's end are zeroed. Waves
+  // past the last segment load segment 0 and discard it.
+  const int64_t gl = live ? g0 : 0;
+  int c[8];
+  Q v[2];
 #pragma unroll
-  for (int it = 0; it < kSegE; ++it) {
-    const int64_t g = g0 + it * 64 + lane;
-    A v = 0;
-    if (g < g1) v = A(ldm(av, g)) * A(ldx(x, ldi(aj, g)));
-    p[it * 64 + lane] = v;
+  for (int h = 0; h < 2; ++h) {
+    const int64_t g = gl + h * 256 + 4 * lane;
+    const int4v cc = __builtin_nontemporal_load(reinterpret_cast<const int4v *>(aj + g));
+    v[h] = __builtin_nontemporal_load(reinterpret_cast<const Q *>(av + g));
+    c[4 * h + 0] = cc.x; c[4 * h + 1] = cc.y; c[4 * h + 2] = cc.z; c[4 * h + 3] = cc.w;
   }
+  // segment bounds and the first 128 owned row pointers (clamped to the
+  // owned range) go out while the matrix stream is in flight
+  const int64_t sl = live ? s : 0;
+  const int f0 = first_row[sl], f1 = first_row[sl + 1];
+  V xv[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) xv[e] = ldx(x, c[e]);
+  const int pa = ai[f0 + lane <= f1 ? f0 + lane : f1];
+  const int pb = ai[f0 + 64 + lane <= f1 ? f0 + 64 + lane : f1];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t g = g0 + h * 256 + 4 * lane + e;
+      const A prod = widen<A>(v[h][e]) * A(xv[4 * h + e]);
+      p[h * 256 + 4 * lane + e] = g < g1 ? prod : A(0);
+    }
   __syncthreads();
   if (!live) return;
   // head: the row that started in an earlier segment (ALBUS bv[2t])
@@ -241,9 +277,8 @@ void launch_seg(Mat &M, const T *av, const V *x, V *y, const V *z, A alpha, A be
 template <typename T, typename V, typename A>
 void dispatch_alg(Mat &M, la_spmv_alg alg, const void *av, const void *x, void *y,
                   const void *z, double alpha, double beta, hipStream_t s) {
-  if (alg == LA_SPMV_AUTO)
-    alg = (M.max_row > 8 * std::max(1.0, M.avg_row) && M.max_row > 64) ? LA_SPMV_ALBUS
-                                                                         : LA_SPMV_ROW;
+  if (alg == LA_SPMV_AUTO) alg = LA_SPMV_ALBUS;
+  if (M.nnz == 0 || M.cols == 0) alg = LA_SPMV_ROW;  // nothing to gather
   const T *a = static_cast<const T *>(av);
   const V *xx = static_cast<const V *>(x);
   V *yy = static_cast<V *>(y);
@@ -285,9 +320,12 @@ std::unique_ptr<Mat> make_mat(int rows, int cols, int base, const int *ai, const
   M->max_row = max_row;
   M->avg_row = rows ? static_cast<double>(nnz) / rows : 0.0;
   M->ai.alloc(sizeof(int) * (static_cast<size_t>(rows) + 1));
-  M->aj.alloc(sizeof(int) * static_cast<size_t>(nnz));
+  const int64_t pnnz = padded_count(nnz);
+  M->aj.alloc(sizeof(int) * static_cast<size_t>(pnnz));
   M->ai.upload(M->h_ai.data(), sizeof(int) * (rows + 1), s);
   M->aj.upload(M->h_aj.data(), sizeof(int) * nnz, s);
+  if (pnnz > nnz)
+    LA_HIP(hipMemsetAsync(M->aj.as<int>() + nnz, 0, sizeof(int) * (pnnz - nnz), s));
   M->vals.set_from_host(av, nnz, s);
   LA_HIP(hipStreamSynchronize(s));
   return M;

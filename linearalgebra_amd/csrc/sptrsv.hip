@@ -30,7 +30,6 @@ using namespace dev;
 
 namespace {
 constexpr int kB = 256;
-constexpr unsigned kSpinLimit = 1u << 22;
 
 template <typename T>
 __device__ __forceinline__ auto ldt(const T *p, int64_t i) {
@@ -60,58 +59,99 @@ __global__ __launch_bounds__(kB) void k_trsv_level(int k0, int k1, const int *__
   }
 }
 
+// x <- sentinel everywhere; ctl[0] (ticket) and ctl[1] (owning XCD + 1)
+// reset for the next sync-free solve.
 template <typename V>
-__global__ void k_fill_sentinel(int64_t n, V *x, unsigned *counter) {
+__global__ void k_fill_sentinel(int64_t n, V *x, unsigned *ctl) {
   using U = typename Sentinel<V>::U;
   U *xb = reinterpret_cast<U *>(x);
   for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
        t += static_cast<int64_t>(gridDim.x) * kB)
     xb[t] = Sentinel<V>::bits;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 2) ctl[threadIdx.x] = 0;
+}
+
+// Sync-free solve confined to one XCD. The first workgroup to start claims
+// its XCD (agent-scope CAS on ctl[1]); workgroups that land on another XCD
+// leave at once. The rest claim chunk tickets from ctl[0] in level order and
+// poll readiness through the XCD's L2 (ld_l2_*), publishing with plain
+// stores that write through to that L2. A wait longer than kWaitTicks of
+// the 100 MHz real-time clock sets *err (device scope) and gives up; every
+// other waiter checks *err every 256 polls and gives up too, so the kernel
+// always drains quickly.
+constexpr uint64_t kWaitTicks = 20000000;  // 0.2 s
+
+template <typename V>
+__device__ __forceinline__ typename Sentinel<V>::U ld_ready_l2(const V *p) {
+  if constexpr (sizeof(V) == 8) return ld_l2_u64(p);
+  else return ld_l2_u32(p);
 }
 
 template <int G, typename T, typename V, typename A, bool DIAG>
 __global__ __launch_bounds__(kB) void k_trsv_syncfree(
     int nchunks, const int *__restrict__ chunk_start, const int *__restrict__ rowptr,
     const int *__restrict__ col, const T *__restrict__ val, const T *__restrict__ diag,
-    const int *__restrict__ perm, const V *__restrict__ b, V *x, unsigned *counter,
-    int *err) {
+    const int *__restrict__ perm, const V *__restrict__ b, V *x, unsigned *ctl, int *err) {
+  using U = typename Sentinel<V>::U;
+  __shared__ int s_go;
+  if (threadIdx.x == 0) {
+    const unsigned me = xcc_id() + 1;
+    const unsigned prev = atomicCAS(&ctl[1], 0u, me);
+    s_go = (prev == 0u || prev == me) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_go) return;
   const int lane = threadIdx.x & 63;
   const int gi = lane / G, sub = lane & (G - 1);
   for (;;) {
     int c = 0;
-    if (lane == 0) c = static_cast<int>(atomicAdd(counter, 1u));
+    if (lane == 0)
+      c = static_cast<int>(__hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP));
     c = __shfl(c, 0, 64);
     if (c >= nchunks) break;
     const int k0 = chunk_start[c], k1 = chunk_start[c + 1];
     const int k = k0 + gi;
     if (k < k1) {
+      // everything that does not depend on x first
       const int rs = rowptr[k], re = rowptr[k + 1];
+      const int r = perm[k];
+      const A bb = A(b[r]);
+      A dd = A(1);
+      if constexpr (DIAG) dd = A(ldt(diag, k));
       A acc = 0;
-      bool timed_out = false;
+      bool gave_up = false;
       for (int q = rs + sub; q < re; q += G) {
         const int j = ldi(col, q);
         const A a = A(ldt(val, q));
-        V xv = load_ready(x + j);
-        unsigned spins = 0;
-        while (is_sentinel(xv)) {
-          __builtin_amdgcn_s_sleep(1);
-          xv = load_ready(x + j);
-          if (++spins > kSpinLimit) {
-            timed_out = true;
-            xv = V(0);
-            break;
+        U u = ld_ready_l2(x + j);
+        if (u == Sentinel<V>::bits && !gave_up) {
+          const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+          unsigned polls = 0;
+          while (u == Sentinel<V>::bits) {
+            __builtin_amdgcn_s_sleep(1);
+            u = ld_ready_l2(x + j);
+            if ((++polls & 255u) == 0) {
+              if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                  __builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                gave_up = true;
+                break;
+              }
+            }
           }
         }
-        acc += a * A(xv);
+        acc += a * A(u == Sentinel<V>::bits ? V(0) : from_bits<V>(u));
       }
-      if (timed_out) atomicOr(err, 1);
+      if (gave_up)
+        __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       acc = group_sum<G>(acc);
       if (sub == 0) {
-        const int r = perm[k];
-        A v = A(b[r]) - acc;
-        if constexpr (DIAG) v = v / A(ldt(diag, k));
-        publish(x + r, V(v));
+        A v = bb - acc;
+        if constexpr (DIAG) v = v / dd;
+        U o = to_bits<V>(V(v));
+        if (o == Sentinel<V>::bits) o = Sentinel<V>::qnan;
+        __hip_atomic_store(reinterpret_cast<U *>(x + r), o, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     }
   }
@@ -155,10 +195,10 @@ void launch_syncfree(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x,
   if (fg == 0) fg = 1;
   hipLaunchKernelGGL(k_fill_sentinel<V>, dim3(fg), dim3(kB), 0, s, n, x, P.counter.as<unsigned>());
   LA_HIP(hipGetLastError());
-  // persistent-style grid: at most what the chip keeps resident
-  const unsigned waves_needed = static_cast<unsigned>(P.nchunks);
-  unsigned grid = std::min<unsigned>((waves_needed + 3) / 4, 256u * 8u);
-  if (grid == 0) grid = 1;
+  // one XCD does the solve: 8x the workgroups it needs (at most the 256 one
+  // XCD keeps resident), the others exit at once
+  const unsigned per_xcd = std::min<unsigned>((static_cast<unsigned>(P.nchunks) + 3) / 4, 256u);
+  const unsigned grid = 8u * std::max(1u, per_xcd);
 #define LA_SF(GG)                                                                              \
   hipLaunchKernelGGL((k_trsv_syncfree<GG, T, V, A, DIAG>), dim3(grid), dim3(kB), 0, s,        \
                      P.nchunks, P.chunk_start.as<int>(), P.rowptr.as<int>(), P.col.as<int>(), \

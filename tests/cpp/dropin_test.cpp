@@ -1,0 +1,132 @@
+// Drop-in C++ layer test, written the way the reference's gtest files use
+// the same interfaces (tests/spmv_test.cpp, tests/triangular_solve_test.cpp,
+// execs/linearSolve.cpp): SpMV functors, SPMV<>, OptimizedTriangularSolve,
+// IC(0)/ILU(k) preconditioners and the PCG/FGMRES solvers on a 2-D Poisson
+// matrix, against plain serial loops. Prints PASS/FAIL per check; exit code
+// is the number of failures.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "laamd/dropin.hpp"
+
+using namespace laamd;
+static int failures = 0;
+static void expect(bool ok, const char *name, double v) {
+  std::printf("%s %s (%.3e)\n", ok ? "PASS" : "FAIL", name, v);
+  if (!ok) ++failures;
+}
+
+static double rel(const std::vector<double> &a, const std::vector<double> &b) {
+  double num = 0, den = 0;
+  for (size_t i = 0; i < a.size(); ++i) {
+    num += (a[i] - b[i]) * (a[i] - b[i]);
+    den += b[i] * b[i];
+  }
+  return std::sqrt(num / (den > 0 ? den : 1));
+}
+
+int main() {
+  const int nx = 48, n = nx * nx;
+  std::vector<int> ai(n + 1);
+  int64_t nnz = 0;
+  detail::check(la_gen_poisson2d(nx, nx, ai.data(), nullptr, nullptr, &nnz));
+  std::vector<int> aj(nnz);
+  std::vector<double> av(nnz);
+  detail::check(la_gen_poisson2d(nx, nx, ai.data(), aj.data(), av.data(), &nnz));
+  std::vector<double> b(n), ref(n), x(n);
+  for (int i = 0; i < n; ++i) b[i] = std::sin(0.1 * i);
+  for (int i = 0; i < n; ++i) {
+    double s = 0;
+    for (int k = ai[i]; k < ai[i + 1]; ++k) s += av[k] * b[aj[k]];
+    ref[i] = s;
+  }
+  // SpMV functors (spmv.hpp)
+  matrix_utils::SerialSPMV serial;
+  serial(n, 0, ai.data(), aj.data(), av.data(), b.data(), x.data(), 1., 0.);
+  expect(rel(x, ref) < 1e-14, "SerialSPMV", rel(x, ref));
+  matrix_utils::ALBUSSPMV<> albus(4);
+  albus.preprocess(n, 0, ai.data(), aj.data(), av.data());
+  albus(n, 0, ai.data(), aj.data(), av.data(), b.data(), x.data(), 1., 0.);
+  expect(rel(x, ref) < 1e-14, "ALBUSSPMV", rel(x, ref));
+  matrix_utils::CSRMatrixVec<int, int, double> M;
+  M.rows = M.cols = n;
+  M.ai = ai;
+  M.aj = aj;
+  M.av = av;
+  matrix_utils::SPMV<decltype(M), matrix_utils::SegSumSPMV<>> spmv;
+  spmv.setMatrix(&M);
+  spmv.preprocess();
+  std::vector<double> y(n, 1.0);
+  spmv(b.data(), y.data(), 2.0, -1.0);
+  std::vector<double> ref2(n);
+  for (int i = 0; i < n; ++i) ref2[i] = 2.0 * ref[i] - 1.0;
+  expect(rel(y, ref2) < 1e-14, "SPMV<SegSum> alpha/beta", rel(y, ref2));
+
+  // triangular solves (triangle_solve.hpp) on L/D/U of A
+  matrix_utils::CSRMatrixVec<int, int, double> Lm, Um;
+  std::vector<double> D;
+  matrix_utils::SplitLDU(n, 0, ai.data(), aj.data(), av.data(), Lm, D, Um);
+  std::vector<double> xs(n), xl(n);
+  for (int i = 0; i < n; ++i) {
+    double s = 0;
+    for (int k = Lm.ai[i]; k < Lm.ai[i + 1]; ++k) s += Lm.av[k] * xs[Lm.aj[k]];
+    xs[i] = b[i] - s;
+  }
+  matrix_utils::OptimizedTriangularSolve<matrix_utils::FBSubstitutionType::Barrier,
+                                         matrix_utils::L>
+      fw;
+  fw.analysis(n, 0, Lm.AI(), Lm.AJ(), Lm.AV());
+  for (int rep = 0; rep < 3; ++rep) fw(b.data(), xl.data());
+  expect(rel(xl, xs) < 1e-14, "OptimizedTriangularSolve<Barrier,L>", rel(xl, xs));
+  expect(fw.levels() == 2 * nx - 1, "levels == 2nx-1", fw.levels());
+  matrix_utils::OptimizedTriangularSolve<matrix_utils::FBSubstitutionType::NoBarrier,
+                                         matrix_utils::U>
+      bw;
+  bw.analysis(n, 0, Um.AI(), Um.AJ(), Um.AV(), D.data());
+  for (int i = n - 1; i >= 0; --i) {
+    double s = 0;
+    for (int k = Um.ai[i]; k < Um.ai[i + 1]; ++k) s += Um.av[k] * xs[Um.aj[k]];
+    xs[i] = (b[i] - s) / D[i];
+  }
+  bw(b.data(), xl.data());
+  expect(rel(xl, xs) < 1e-14, "OptimizedTriangularSolve<NoBarrier,U>", rel(xl, xs));
+
+  // IC(0)-PCG as execs/linearSolve.cpp builds it
+  mkl_wrapper::mkl_sparse_mat A(n, n, ai, aj, av);
+  mkl_wrapper::mkl_sparse_mat_sym Asym(A);
+  auto prec = std::make_shared<mkl_wrapper::incomplete_cholesky_k>();
+  prec->set_level(0);
+  prec->shift(true);
+  bool ok = prec->symbolic_factorize(&Asym) && prec->numeric_factorize(&Asym);
+  expect(ok, "incomplete_cholesky_k factorize", 0);
+  mkl_wrapper::mkl_pcg_solver pcg(&Asym, prec);
+  pcg.set_rel_tol(1e-10);
+  pcg.set_max_iters(1000);
+  std::vector<double> sol(n, 0.0), rhs(ref);
+  ok = pcg.solve(rhs.data(), sol.data());
+  expect(ok && rel(sol, b) < 1e-7, "mkl_pcg_solver + IC(0)", rel(sol, b));
+  expect(pcg.last_result().rel_residual < 1e-10, "PCG true residual", pcg.last_result().rel_residual);
+
+  // ILU(1)-FGMRES through the factory, the "gmres+iluk" registration of
+  // execs/test1.cpp:61-77
+  auto &factory = utils::singleton<mkl_wrapper::solver_factory>::instance();
+  std::shared_ptr<mkl_wrapper::incomplete_lu_k> ilu;
+  factory.reg("gmres+iluk", [&](mkl_wrapper::mkl_sparse_mat &m) {
+    ilu = std::make_shared<mkl_wrapper::incomplete_lu_k>();
+    ilu->set_level(1);
+    ilu->symbolic_factorize(&m);
+    ilu->numeric_factorize(&m);
+    auto s = std::make_unique<mkl_wrapper::mkl_fgmres_solver>(&m, ilu);
+    s->set_rel_tol(1e-10);
+    s->set_restart_steps(30);
+    return mkl_wrapper::solver_factory::solver_ptr(std::move(s));
+  });
+  auto gm = factory.create("gmres+iluk", A);
+  std::fill(sol.begin(), sol.end(), 0.0);
+  ok = gm && gm->solve(rhs.data(), sol.data());
+  expect(ok && rel(sol, b) < 1e-7, "factory gmres+iluk", rel(sol, b));
+  expect(factory.create("no-such-solver", A) == nullptr, "factory unknown name", 0);
+  std::printf("%d failure(s)\n", failures);
+  return failures;
+}
