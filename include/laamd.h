@@ -291,6 +291,13 @@ int la_pcg_solve(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
  * (flexible) preconditioned GMRES(restart). */
 int la_fgmres_solve(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
                     const double *b, double *x, la_solver_result *res);
+/* A^T x = b with the same loops, the transposed product
+ * (mkl_sparse_mat::transpose_mult_vec, mkl_sparse_mat.cpp:331-336) and the
+ * transposed preconditioner apply M^-T; the adjoint solve of x = A^-1 b. */
+int la_pcg_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
+                           const double *b, double *x, la_solver_result *res);
+int la_fgmres_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
+                              const double *b, double *x, la_solver_result *res);
 
 #ifdef __cplusplus
 }

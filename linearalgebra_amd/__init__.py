@@ -263,6 +263,15 @@ def fgmres(A, P, b, x, raise_on_fail=False, **kw):
     return _solve(lib().la_fgmres_solve, A, P, b, x, raise_on_fail, kw)
 
 
+def pcg_transpose(A, P, b, x, raise_on_fail=False, **kw):
+    """A^T x = b (transposed product and preconditioner apply)."""
+    return _solve(lib().la_pcg_solve_transpose, A, P, b, x, raise_on_fail, kw)
+
+
+def fgmres_transpose(A, P, b, x, raise_on_fail=False, **kw):
+    return _solve(lib().la_fgmres_solve_transpose, A, P, b, x, raise_on_fail, kw)
+
+
 # ---------------------------------------------------------------- host
 
 

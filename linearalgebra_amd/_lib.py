@@ -103,6 +103,10 @@ _SIGS = {
     "la_solver_opts_default": [C.POINTER(SolverOpts)],
     "la_pcg_solve": [_vp, _vp, C.POINTER(SolverOpts), _vp, _vp, C.POINTER(SolverResult)],
     "la_fgmres_solve": [_vp, _vp, C.POINTER(SolverOpts), _vp, _vp, C.POINTER(SolverResult)],
+    "la_pcg_solve_transpose": [_vp, _vp, C.POINTER(SolverOpts), _vp, _vp,
+                               C.POINTER(SolverResult)],
+    "la_fgmres_solve_transpose": [_vp, _vp, C.POINTER(SolverOpts), _vp, _vp,
+                                  C.POINTER(SolverResult)],
 }
 _RESTYPES = {"la_last_error": C.c_char_p, "la_solver_opts_default": None}
 

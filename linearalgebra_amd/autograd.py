@@ -1,0 +1,100 @@
+"""Adjoint of x = A(vals)^{-1} b as a torch.autograd.Function -- a TEST
+BINDING over the C ABI, not part of the hot path.
+
+forward: the CSR values `vals` (FP64, device) are written into the device
+matrix, the preconditioner is refactored on the host (IC(0) or ILU(k)
+numeric phase) and uploaded, and the system is solved by PCG (SPD) or
+FGMRES through the C ABI.
+backward: the adjoint system A^T lambda = dL/dx is solved with the
+transposed product and the transposed preconditioner apply
+(la_pcg_solve_transpose / la_fgmres_solve_transpose); then
+dL/db = lambda and dL/dvals[k] = -lambda[row k] * x[col k]
+(la_csr_value_grad), the gradient of -lambda^T A x on the sparsity pattern.
+"""
+import numpy as np
+import torch
+
+import linearalgebra_amd as la
+
+
+class SparseSystem:
+    """Pattern, solver settings and device handles shared by forward and
+    backward. `kind` is "ic" (symmetric positive definite, PCG) or "ilu"
+    (FGMRES). `precision` is one of la.PREC_*."""
+
+    def __init__(self, A, kind="ic", level=0, precision=la.PREC_FP64, rel_tol=1e-12,
+                 max_iters=5000, trsv_mode=la.TRSV_SYNCFREE, inner_tol=1e-5):
+        self.A = la.CSR(A.n, A.ai, A.aj, A.av)
+        self.kind, self.level = kind, level
+        self.dev = la.DeviceMatrix(self.A)
+        self.opts = dict(rel_tol=rel_tol, max_iters=max_iters, trsv_mode=trsv_mode,
+                         precision=precision, inner_tol=inner_tol)
+        self.P = None
+        self.last = {}
+        if kind == "ic":
+            self.U_pattern = la.split_triangle(self.A, upper=True)
+            self.diag_pos = self.U_pattern.ai[:-1]
+            self.icc_ai, self.icc_aj = la.icc_symbolic(self.U_pattern, self.diag_pos, level)
+        else:
+            self.fai, self.faj, self.fu = la.iluk_symbolic(self.A, level)
+
+    def refactor(self, vals_host):
+        self.A.av = np.ascontiguousarray(vals_host, dtype=np.float64)
+        if self.kind == "ic":
+            U = la.split_triangle(self.A, upper=True)
+            R, ok = la.icc_numeric(U, self.diag_pos, self.icc_ai, self.icc_aj)
+            if not ok:
+                raise la.LaError(5, "IC factorisation hit a negative pivot")
+            self.P = la.Preconditioner(R, "ic")
+        else:
+            F = la.iluk_numeric(self.A, self.fai, self.faj, self.fu)
+            self.P = la.Preconditioner(F, "ilu")
+
+    def solve(self, b, transpose=False):
+        x = torch.zeros_like(b)
+        if self.kind == "ic":
+            fn = la.pcg_transpose if transpose else la.pcg
+        else:
+            fn = la.fgmres_transpose if transpose else la.fgmres
+        res = fn(self.dev, self.P, b, x, raise_on_fail=True, **self.opts)
+        self.last["adjoint" if transpose else "forward"] = res
+        return x
+
+
+class SparseSolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vals, b, system):
+        system.dev.set_values(vals.detach().contiguous())
+        system.refactor(vals.detach().cpu().numpy())
+        x = system.solve(b.detach().contiguous())
+        ctx.system = system
+        ctx.save_for_backward(x)
+        return x
+
+    @staticmethod
+    def backward(ctx, grad_x):
+        (x,) = ctx.saved_tensors
+        S = ctx.system
+        lam = S.solve(grad_x.contiguous(), transpose=True)
+        gvals = torch.zeros(S.A.nnz, dtype=torch.float64, device=x.device)
+        S.dev.value_grad(lam, x, gvals)
+        return gvals, lam, None
+
+
+def sparse_solve(vals, b, system):
+    """x = A(vals)^{-1} b, differentiable in vals and b."""
+    return SparseSolve.apply(vals, b, system)
+
+
+def edge_values(theta, base_vals, val_map):
+    """CSR values of the 2-D edge-conductivity diffusion operator as a
+    differentiable function of the edge conductivities theta:
+    A_ij = A_ji = -theta_e, A_ii and A_jj gain +theta_e; base_vals holds the
+    theta-independent part (the Dirichlet ground terms on the diagonal)."""
+    vm = val_map
+    v = base_vals.clone()
+    v = v.index_add(0, vm[:, 0], -theta)
+    v = v.index_add(0, vm[:, 1], -theta)
+    v = v.index_add(0, vm[:, 2], theta)
+    v = v.index_add(0, vm[:, 3], theta)
+    return v

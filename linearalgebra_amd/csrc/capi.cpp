@@ -12,6 +12,10 @@ int pcg_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double 
               la_solver_result *res);
 int fgmres_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
                  la_solver_result *res);
+int pcg_solve_transpose(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
+                        la_solver_result *res);
+int fgmres_solve_transpose(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
+                           la_solver_result *res);
 void gen_uniform(int64_t n, uint64_t seed, double lo, double hi, double *out);
 void gen_lognormal(int64_t n, uint64_t seed, double mu, double sigma, double *out);
 void gen_poisson2d(int nx, int ny, int *ai, int *aj, double *av, int64_t *nnz);
@@ -530,6 +534,32 @@ int la_fgmres_solve(la_mat_t A, la_prec_t P, const la_solver_opts *opts, const d
   const int st = fgmres_solve(*A->m, P ? P->p.get() : nullptr, *opts, b, x, res);
   if (st != LA_OK) {
     set_last_error("FGMRES did not reach the tolerance within the iteration limit");
+    return st;
+  }
+  LA_CATCH
+}
+
+int la_pcg_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts, const double *b,
+                           double *x, la_solver_result *res) {
+  LA_TRY
+  LA_CHECK_ARG(A && opts && b && x, "null argument");
+  if (P) P->p->lower->stream = P->p->upper->stream = A->m->stream;
+  const int st = pcg_solve_transpose(*A->m, P ? P->p.get() : nullptr, *opts, b, x, res);
+  if (st != LA_OK) {
+    set_last_error("PCG (transposed) did not reach the tolerance within the iteration limit");
+    return st;
+  }
+  LA_CATCH
+}
+
+int la_fgmres_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
+                              const double *b, double *x, la_solver_result *res) {
+  LA_TRY
+  LA_CHECK_ARG(A && opts && b && x, "null argument");
+  if (P) P->p->lower->stream = P->p->upper->stream = A->m->stream;
+  const int st = fgmres_solve_transpose(*A->m, P ? P->p.get() : nullptr, *opts, b, x, res);
+  if (st != LA_OK) {
+    set_last_error("FGMRES (transposed) did not reach the tolerance within the iteration limit");
     return st;
   }
   LA_CATCH

@@ -37,6 +37,9 @@ void spmv(Mat &A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, dou
           const void *x, double beta, const void *z, void *y, hipStream_t s);
 void spmv_transpose(Mat &A, la_dtype val, la_dtype vec, double alpha, const void *x,
                     double beta, void *y, hipStream_t s);
+// A^T as its own matrix (host stable transpose of the FP64 values, built once;
+// lower-precision copies follow A's)
+Mat &ensure_transposed(Mat &A, hipStream_t s);
 void csr_value_grad(Mat &A, const double *lambda, const double *x, double *g, hipStream_t s);
 
 // -------------------------------------------------------------- SpTRSV plan

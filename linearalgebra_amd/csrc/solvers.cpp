@@ -61,10 +61,14 @@ la_dtype inner_val_dtype(int precision) {
   return precision == LA_PREC_MIXED_BF16 ? LA_BF16 : LA_F32;
 }
 
+// z = M^-1 r, or M^-T r when the solve is with A^T
+thread_local bool tl_transpose = false;
 void apply_M(Prec *P, const la_solver_opts &o, la_dtype val, la_dtype vec, const void *r,
              void *z, int64_t n, hipStream_t s) {
-  if (P) prec_apply(*P, static_cast<la_trsv_mode>(o.trsv_mode), val, vec, r, z, s);
-  else copy_vec(n, vec, r, vec, z, 1.0, s);
+  const la_trsv_mode m = static_cast<la_trsv_mode>(o.trsv_mode);
+  if (!P) copy_vec(n, vec, r, vec, z, 1.0, s);
+  else if (tl_transpose) prec_apply_transpose(*P, m, val, vec, r, z, s);
+  else prec_apply(*P, m, val, vec, r, z, s);
 }
 
 // One PCG run on (val, vec) precision. x holds x0; vectors b/x in `vec`.
@@ -285,6 +289,14 @@ int run_solver(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double
 }
 }  // namespace
 
+// A^T x = b with the same loops: the product uses the transposed matrix
+// (built once, stable transpose) and the preconditioner its transposed apply.
+struct TransposeScope {
+  explicit TransposeScope(bool on) { tl_transpose = on; }
+  ~TransposeScope() { tl_transpose = false; }
+};
+Mat &transposed_of(Mat &A);
+
 int pcg_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
               la_solver_result *res) {
   LA_CHECK_ARG(A.rows == A.cols, "PCG needs a square matrix");
@@ -311,4 +323,22 @@ int fgmres_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, doub
                     });
 }
 
+}  // namespace laamd
+
+namespace laamd {
+Mat &transposed_of(Mat &A) { return ensure_transposed(A, A.stream); }
+
+int pcg_solve_transpose(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
+                        la_solver_result *res) {
+  Mat &At = transposed_of(A);
+  TransposeScope scope(true);
+  return pcg_solve(At, P, o, b, x, res);
+}
+
+int fgmres_solve_transpose(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
+                           la_solver_result *res) {
+  Mat &At = transposed_of(A);
+  TransposeScope scope(true);
+  return fgmres_solve(At, P, o, b, x, res);
+}
 }  // namespace laamd

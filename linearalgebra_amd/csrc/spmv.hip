@@ -373,8 +373,7 @@ void spmv(Mat &A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, dou
   }
 }
 
-void spmv_transpose(Mat &A, la_dtype val, la_dtype vec, double alpha, const void *x,
-                    double beta, void *y, hipStream_t s) {
+Mat &ensure_transposed(Mat &A, hipStream_t s) {
   if (!A.transposed) {
     // download the FP64 values, transpose on the host (stable), upload
     std::vector<double> hv(static_cast<size_t>(A.nnz));
@@ -388,7 +387,16 @@ void spmv_transpose(Mat &A, la_dtype val, la_dtype vec, double alpha, const void
               tv.data());
     A.transposed = make_mat(A.cols, A.rows, 0, ti.data(), tj.data(), tv.data(), s);
   }
-  Mat &T = *A.transposed;
+  A.transposed->stream = A.stream;
+  for (int dt = 1; dt < 3; ++dt)
+    if (A.vals.has[dt] && !A.transposed->vals.has[dt])
+      A.transposed->vals.add(static_cast<la_dtype>(dt), s);
+  return *A.transposed;
+}
+
+void spmv_transpose(Mat &A, la_dtype val, la_dtype vec, double alpha, const void *x,
+                    double beta, void *y, hipStream_t s) {
+  Mat &T = ensure_transposed(A, s);
   const la_dtype acc = (val == LA_F64 && vec == LA_F64) ? LA_F64 : LA_F32;
   spmv(T, LA_SPMV_AUTO, val, vec, acc, alpha, x, beta, y, y, s);
 }
