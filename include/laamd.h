@@ -267,7 +267,7 @@ typedef struct {
   int restart;         /* FGMRES: set_restart_steps (default 30)           */
   int stop_rule;       /* la_stop_rule                                      */
   int precision;       /* la_precision                                     */
-  int trsv_mode;       /* la_trsv_mode for the preconditioner apply        */
+  int trsv_mode;       /* la_trsv_mode for the preconditioner apply (default SYNCFREE) */
   int spmv_alg;        /* la_spmv_alg                                      */
   double inner_tol;    /* mixed: inner relative reduction per refinement   */
   int max_outer;       /* mixed: refinement steps                          */

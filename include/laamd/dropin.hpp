@@ -512,7 +512,7 @@ class device_preconditioner : public mkl_sparse_mat {
     P_ = std::shared_ptr<la_prec_s>(p, [](la_prec_t q) { if (q) la_prec_destroy(q); });
   }
   std::shared_ptr<la_prec_s> P_;
-  la_trsv_mode trsv_mode_ = LA_TRSV_LEVEL;
+  la_trsv_mode trsv_mode_ = LA_TRSV_SYNCFREE;
 };
 
 // incomplete_lu_k (mkl_wrapper/incomplete_lu.h:55-70, .cpp:153-325)

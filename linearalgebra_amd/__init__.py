@@ -176,13 +176,13 @@ class TriangularSolve:
     def add_dtype(self, dt):
         check(lib().la_trsv_add_dtype(self.h, dt))
 
-    def solve(self, b, x, mode=TRSV_LEVEL, val=None):
+    def solve(self, b, x, mode=TRSV_SYNCFREE, val=None):
         vec = _vec_code(b)
         val = vec if val is None else val
         check(lib().la_trsv_solve(self.h, mode, val, vec, _dptr(b), _dptr(x)))
         return x
 
-    def solve_transpose(self, b, x, mode=TRSV_LEVEL, val=None):
+    def solve_transpose(self, b, x, mode=TRSV_SYNCFREE, val=None):
         vec = _vec_code(b)
         val = vec if val is None else val
         check(lib().la_trsv_solve_transpose(self.h, mode, val, vec, _dptr(b), _dptr(x)))
@@ -219,13 +219,13 @@ class Preconditioner:
         ptr = stream if isinstance(stream, int) else stream.cuda_stream
         check(lib().la_prec_set_stream(self.h, C.c_void_p(ptr)))
 
-    def apply(self, r, z, mode=TRSV_LEVEL, val=None):
+    def apply(self, r, z, mode=TRSV_SYNCFREE, val=None):
         vec = _vec_code(r)
         val = vec if val is None else val
         check(lib().la_prec_apply(self.h, mode, val, vec, _dptr(r), _dptr(z)))
         return z
 
-    def apply_transpose(self, r, z, mode=TRSV_LEVEL, val=None):
+    def apply_transpose(self, r, z, mode=TRSV_SYNCFREE, val=None):
         vec = _vec_code(r)
         val = vec if val is None else val
         check(lib().la_prec_apply_transpose(self.h, mode, val, vec, _dptr(r), _dptr(z)))

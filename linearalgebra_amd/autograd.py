@@ -20,15 +20,21 @@ import linearalgebra_amd as la
 class SparseSystem:
     """Pattern, solver settings and device handles shared by forward and
     backward. `kind` is "ic" (symmetric positive definite, PCG) or "ilu"
-    (FGMRES). `precision` is one of la.PREC_*."""
+    (FGMRES). `precision` is one of la.PREC_*. `adjoint_rel_tol` (default
+    `rel_tol`) is the relative-residual target of the backward solve: an
+    adjoint right-hand side such as 2(x - x*) is smooth, and for smooth
+    right-hand sides the attainable FP64 relative residual is about
+    eps * cond(A), above a forward tolerance that a rough b still reaches."""
 
     def __init__(self, A, kind="ic", level=0, precision=la.PREC_FP64, rel_tol=1e-12,
-                 max_iters=5000, trsv_mode=la.TRSV_SYNCFREE, inner_tol=1e-5):
+                 max_iters=5000, trsv_mode=la.TRSV_SYNCFREE, inner_tol=1e-5,
+                 adjoint_rel_tol=None):
         self.A = la.CSR(A.n, A.ai, A.aj, A.av)
         self.kind, self.level = kind, level
         self.dev = la.DeviceMatrix(self.A)
         self.opts = dict(rel_tol=rel_tol, max_iters=max_iters, trsv_mode=trsv_mode,
                          precision=precision, inner_tol=inner_tol)
+        self.adjoint_rel_tol = rel_tol if adjoint_rel_tol is None else adjoint_rel_tol
         self.P = None
         self.last = {}
         if kind == "ic":
@@ -56,7 +62,10 @@ class SparseSystem:
             fn = la.pcg_transpose if transpose else la.pcg
         else:
             fn = la.fgmres_transpose if transpose else la.fgmres
-        res = fn(self.dev, self.P, b, x, raise_on_fail=True, **self.opts)
+        opts = dict(self.opts)
+        if transpose:
+            opts["rel_tol"] = self.adjoint_rel_tol
+        res = fn(self.dev, self.P, b, x, raise_on_fail=True, **opts)
         self.last["adjoint" if transpose else "forward"] = res
         return x
 

@@ -506,7 +506,7 @@ void la_solver_opts_default(la_solver_opts *o) {
   o->restart = 30;
   o->stop_rule = LA_STOP_TRUE_RESIDUAL;
   o->precision = LA_PREC_FP64;
-  o->trsv_mode = LA_TRSV_LEVEL;
+  o->trsv_mode = LA_TRSV_SYNCFREE;
   o->spmv_alg = LA_SPMV_AUTO;
   o->inner_tol = 1e-5;
   o->max_outer = 20;

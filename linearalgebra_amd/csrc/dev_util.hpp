@@ -91,28 +91,9 @@ __device__ __forceinline__ V from_bits(typename Sentinel<V>::U u) {
   else return __uint_as_float(u);
 }
 
-// ---------------------------------------------------------------- XCD-local
-// MI355X has 8 XCDs, each with its own L2. A load with the sc0 bit skips the
-// CU's L1 and is served by the XCD's L2, which every CU of that XCD writes
-// through to; it therefore sees stores made by other CUs of the same XCD,
-// at L2 latency rather than the memory-side round trip a device-scope load
-// needs. Only valid while producer and consumer run on one XCD.
-__device__ __forceinline__ unsigned long long ld_l2_u64(const void *p) {
-  unsigned long long v;
-  asm volatile("global_load_dwordx2 %0, %1, off sc0\n\ts_waitcnt vmcnt(0)"
-               : "=v"(v)
-               : "v"(p)
-               : "memory");
-  return v;
-}
-__device__ __forceinline__ unsigned ld_l2_u32(const void *p) {
-  unsigned v;
-  asm volatile("global_load_dword %0, %1, off sc0\n\ts_waitcnt vmcnt(0)"
-               : "=v"(v)
-               : "v"(p)
-               : "memory");
-  return v;
-}
+// ---------------------------------------------------------------- XCD id
+// MI355X has 8 XCDs, each with its own L2; workgroups are dealt to them
+// round-robin in dispatch order.
 // the XCD this wavefront runs on (HW_REG_XCC_ID, bits 3:0)
 __device__ __forceinline__ unsigned xcc_id() {
   unsigned v;

@@ -53,6 +53,11 @@ struct TrsvPlan {
   std::vector<int> level_ptr;  // host, nlevels+1
   int nchunks = 0;
   DevBuf chunk_start, counter, err_flag;
+  // sync-free launch shape: confined to one XCD (plain stores, L2-latency
+  // polls) or spread over all XCDs; workgroups taking part
+  bool sf_xlocal = true;
+  unsigned sf_wgs = 32;
+  int sf_nt = 256;  // threads per workgroup; a chunk holds sf_nt/G rows
   hipStream_t stream = nullptr;
   // host copy of the strict triangle + diagonal (0-based), for the transpose
   std::vector<int> h_ai, h_aj;

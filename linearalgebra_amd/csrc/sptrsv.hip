@@ -9,17 +9,18 @@
 // Solve, two modes:
 //  * LEVEL: one launch per level (the Barrier mode, triangle_solve.tpp:
 //    127-155), captured once into a HIP graph and replayed.
-//  * SYNCFREE: one launch; wavefronts claim chunks of rows in level order
+//  * SYNCFREE: one launch; workgroups claim chunks of rows in level order
 //    from an atomic ticket (the NoBarrier mode, triangle_solve.tpp:157-187).
 //    x is pre-filled with a sentinel; a row waits until each x_j it reads is
-//    no longer the sentinel (agent-scope relaxed atomic load, value and
-//    readiness in one word) and publishes its own x_i the same way. Chunks
-//    never span levels, so rows inside one wavefront never wait on each
-//    other, and every wait is on an earlier ticket, which a resident
-//    wavefront already holds: the solve always drains. Waits are bounded;
-//    a timeout sets an error flag instead of hanging.
+//    no longer the sentinel (value and readiness in one word) and publishes
+//    its own x_i the same way. Chunks never span levels, so rows inside one
+//    workgroup never wait on each other, and every wait is on an earlier
+//    ticket, which a running workgroup already holds: the solve always
+//    drains. Waits are bounded; a timeout sets an error flag instead of
+//    hanging.
 // Row formula (triangle_solve.hpp:32-71): x_i = (b_i - sum_j T_ij x_j) / d_i.
 #include <algorithm>
+#include <cstdlib>
 
 #include "dev_util.hpp"
 #include "device.hpp"
@@ -71,44 +72,49 @@ __global__ void k_fill_sentinel(int64_t n, V *x, unsigned *ctl) {
   if (blockIdx.x == 0 && threadIdx.x < 2) ctl[threadIdx.x] = 0;
 }
 
-// Sync-free solve confined to one XCD. The first workgroup to start claims
-// its XCD (agent-scope CAS on ctl[1]); workgroups that land on another XCD
-// leave at once. The rest claim chunk tickets from ctl[0] in level order and
-// poll readiness through the XCD's L2 (ld_l2_*), publishing with plain
-// stores that write through to that L2. A wait longer than kWaitTicks of
-// the 100 MHz real-time clock sets *err (device scope) and gives up; every
-// other waiter checks *err every 256 polls and gives up too, so the kernel
-// always drains quickly.
+// Sync-free solve. Readiness is polled with agent-scope relaxed loads (they
+// bypass the CU's L1).
+//  XLOCAL = true: confined to one XCD. The first workgroup to start claims
+//    its XCD (device-coherent CAS on ctl[1]); workgroups that land on another
+//    XCD leave at once. Results are published with plain stores into that
+//    XCD's L2, where the agent-scope polls of the same XCD see them: about
+//    0.3 us from store to poll on MI355X, against 0.55 us when producer and
+//    consumer sit on different XCDs.
+//  XLOCAL = false: every XCD takes part; results are published with
+//    agent-scope stores so polls from any XCD see them.
+// A workgroup claims one chunk ticket from ctl[0] at a time (one device
+// atomic per NT/G rows; the tickets of one counter are handed out about
+// 12 ns apart, so a per-wavefront ticket would bound wide levels). Chunks
+// are in level order. A wait longer than kWaitTicks of the 100 MHz
+// real-time clock sets *err and gives up; every other waiter checks *err
+// every 256 polls and gives up too, so the kernel always drains quickly.
 constexpr uint64_t kWaitTicks = 20000000;  // 0.2 s
 
-template <typename V>
-__device__ __forceinline__ typename Sentinel<V>::U ld_ready_l2(const V *p) {
-  if constexpr (sizeof(V) == 8) return ld_l2_u64(p);
-  else return ld_l2_u32(p);
-}
-
-template <int G, typename T, typename V, typename A, bool DIAG>
-__global__ __launch_bounds__(kB) void k_trsv_syncfree(
+template <int G, int NT, typename T, typename V, typename A, bool DIAG, bool XLOCAL>
+__global__ __launch_bounds__(NT) void k_trsv_syncfree(
     int nchunks, const int *__restrict__ chunk_start, const int *__restrict__ rowptr,
     const int *__restrict__ col, const T *__restrict__ val, const T *__restrict__ diag,
     const int *__restrict__ perm, const V *__restrict__ b, V *x, unsigned *ctl, int *err) {
   using U = typename Sentinel<V>::U;
-  __shared__ int s_go;
-  if (threadIdx.x == 0) {
-    const unsigned me = xcc_id() + 1;
-    const unsigned prev = atomicCAS(&ctl[1], 0u, me);
-    s_go = (prev == 0u || prev == me) ? 1 : 0;
+  if constexpr (XLOCAL) {
+    __shared__ int s_go;
+    if (threadIdx.x == 0) {
+      const unsigned me = xcc_id() + 1;
+      const unsigned prev = atomicCAS(&ctl[1], 0u, me);
+      s_go = (prev == 0u || prev == me) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_go) return;
   }
-  __syncthreads();
-  if (!s_go) return;
-  const int lane = threadIdx.x & 63;
-  const int gi = lane / G, sub = lane & (G - 1);
-  for (;;) {
-    int c = 0;
-    if (lane == 0)
-      c = static_cast<int>(__hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_WORKGROUP));
-    c = __shfl(c, 0, 64);
+  // row of this lane group inside a chunk of NT/G rows
+  const int gi = static_cast<int>(threadIdx.x) / G, sub = threadIdx.x & (G - 1);
+  // two ticket slots: thread 0 writes the next one while slow waves may
+  // still read the current one; one barrier per chunk suffices
+  __shared__ int s_c[2];
+  for (int it = 0;; ++it) {
+    if (threadIdx.x == 0) s_c[it & 1] = static_cast<int>(atomicAdd(&ctl[0], 1u));
+    __syncthreads();
+    const int c = s_c[it & 1];
     if (c >= nchunks) break;
     const int k0 = chunk_start[c], k1 = chunk_start[c + 1];
     const int k = k0 + gi;
@@ -124,13 +130,13 @@ __global__ __launch_bounds__(kB) void k_trsv_syncfree(
       for (int q = rs + sub; q < re; q += G) {
         const int j = ldi(col, q);
         const A a = A(ldt(val, q));
-        U u = ld_ready_l2(x + j);
-        if (u == Sentinel<V>::bits && !gave_up) {
+        V xv = load_ready(x + j);
+        if (is_sentinel(xv) && !gave_up) {
           const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
           unsigned polls = 0;
-          while (u == Sentinel<V>::bits) {
+          while (is_sentinel(xv)) {
             __builtin_amdgcn_s_sleep(1);
-            u = ld_ready_l2(x + j);
+            xv = load_ready(x + j);
             if ((++polls & 255u) == 0) {
               if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
                   __builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
@@ -140,7 +146,7 @@ __global__ __launch_bounds__(kB) void k_trsv_syncfree(
             }
           }
         }
-        acc += a * A(u == Sentinel<V>::bits ? V(0) : from_bits<V>(u));
+        acc += a * A(is_sentinel(xv) ? V(0) : xv);
       }
       if (gave_up)
         __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -150,8 +156,12 @@ __global__ __launch_bounds__(kB) void k_trsv_syncfree(
         if constexpr (DIAG) v = v / dd;
         U o = to_bits<V>(V(v));
         if (o == Sentinel<V>::bits) o = Sentinel<V>::qnan;
-        __hip_atomic_store(reinterpret_cast<U *>(x + r), o, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_WORKGROUP);
+        if constexpr (XLOCAL)
+          __hip_atomic_store(reinterpret_cast<U *>(x + r), o, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+          __hip_atomic_store(reinterpret_cast<U *>(x + r), o, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
       }
     }
   }
@@ -195,15 +205,30 @@ void launch_syncfree(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x,
   if (fg == 0) fg = 1;
   hipLaunchKernelGGL(k_fill_sentinel<V>, dim3(fg), dim3(kB), 0, s, n, x, P.counter.as<unsigned>());
   LA_HIP(hipGetLastError());
-  // one XCD does the solve: 8x the workgroups it needs (at most the 256 one
-  // XCD keeps resident), the others exit at once
-  const unsigned per_xcd = std::min<unsigned>((static_cast<unsigned>(P.nchunks) + 3) / 4, 256u);
-  const unsigned grid = 8u * std::max(1u, per_xcd);
-#define LA_SF(GG)                                                                              \
-  hipLaunchKernelGGL((k_trsv_syncfree<GG, T, V, A, DIAG>), dim3(grid), dim3(kB), 0, s,        \
-                     P.nchunks, P.chunk_start.as<int>(), P.rowptr.as<int>(), P.col.as<int>(), \
-                     val, diag, P.perm.as<int>(), b, x, P.counter.as<unsigned>(),            \
+  // XLOCAL: one XCD does the solve: 8x the workgroups it needs (the others
+  // exit at once); otherwise the whole chip, `sf_wgs` workgroups in total
+  const unsigned need = std::max(1u, static_cast<unsigned>(P.nchunks));
+  const unsigned grid = P.sf_xlocal ? 8u * std::min<unsigned>(need, P.sf_wgs)
+                                    : std::min<unsigned>(need, P.sf_wgs);
+#define LA_SF3(GG, NT, XL)                                                                   \
+  hipLaunchKernelGGL((k_trsv_syncfree<GG, NT, T, V, A, DIAG, XL>), dim3(grid), dim3(NT), 0, s, \
+                     P.nchunks, P.chunk_start.as<int>(), P.rowptr.as<int>(), P.col.as<int>(),  \
+                     val, diag, P.perm.as<int>(), b, x, P.counter.as<unsigned>(),             \
                      P.err_flag.as<int>())
+#define LA_SF2(GG, NT)        \
+  if (P.sf_xlocal) {          \
+    LA_SF3(GG, NT, true);     \
+  } else {                    \
+    LA_SF3(GG, NT, false);    \
+  }
+#define LA_SF(GG)                          \
+  if (P.sf_nt == 1024) {                   \
+    LA_SF2(GG, 1024);                      \
+  } else if (P.sf_nt == 512) {             \
+    LA_SF2(GG, 512);                       \
+  } else {                                 \
+    LA_SF2(GG, 256);                       \
+  }
   switch (P.G) {
     case 1: LA_SF(1); break;
     case 2: LA_SF(2); break;
@@ -213,6 +238,8 @@ void launch_syncfree(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x,
     case 32: LA_SF(32); break;
     default: LA_SF(64); break;
   }
+#undef LA_SF3
+#undef LA_SF2
 #undef LA_SF
   LA_HIP(hipGetLastError());
 }
@@ -303,13 +330,38 @@ std::unique_ptr<TrsvPlan> make_trsv(int n, int base, const int *ai, const int *a
     if (diag) dg[k] = diag[r];
   }
   P->G = pick_G(n ? static_cast<double>(nnz) / n : 1.0);
-  // chunks: 64/G rows, never spanning a level
-  const int R = 64 / P->G;
+  // sync-free launch shape. Narrow levels (few rows per level): the solve is
+  // a chain of dependency hops, so keep it on one XCD, where a hop costs
+  // ~0.3 us instead of ~0.55 us, with 32 workgroups of 256. Wide levels:
+  // the whole chip, 256 workgroups of 512 (fewer tickets per level). The
+  // 1024 rows/level switch point sits between the 512^2 5-point grid (256
+  // rows/level, 6x faster on one XCD than level launches) and the RCM 128^3
+  // 7-point grid (5490 rows/level, 3.3x faster over all XCDs).
+  const double rows_per_level = L > 0 ? static_cast<double>(n) / L : 0.0;
+  if (rows_per_level < 1024.0) {
+    P->sf_xlocal = true;
+    P->sf_wgs = 32;
+    P->sf_nt = 256;
+  } else {
+    P->sf_xlocal = false;
+    P->sf_wgs = 256;
+    P->sf_nt = 512;
+  }
+  // chunks: one workgroup's worth of rows (sf_nt/G), never spanning a level
+  if (const char *e = std::getenv("LAAMD_SF_NT")) {
+    const int v = std::atoi(e);
+    P->sf_nt = v >= 1024 ? 1024 : v >= 512 ? 512 : 256;
+  }
+  const int R = P->sf_nt / P->G;
   std::vector<int> chunks;
   for (int l = 0; l < L; ++l)
     for (int k = P->level_ptr[l]; k < P->level_ptr[l + 1]; k += R) chunks.push_back(k);
   chunks.push_back(n);
   P->nchunks = static_cast<int>(chunks.size()) - 1;
+  // launch-shape overrides for tuning: LAAMD_SF_XLOCAL=0|1, LAAMD_SF_WGS=1..4096
+  if (const char *e = std::getenv("LAAMD_SF_XLOCAL")) P->sf_xlocal = std::atoi(e) != 0;
+  if (const char *e = std::getenv("LAAMD_SF_WGS"))
+    P->sf_wgs = static_cast<unsigned>(std::clamp(std::atoi(e), 1, 4096));
   P->rowptr.alloc(sizeof(int) * (static_cast<size_t>(n) + 1));
   P->col.alloc(sizeof(int) * static_cast<size_t>(nnz));
   P->perm.alloc(sizeof(int) * static_cast<size_t>(std::max(n, 1)));
@@ -395,7 +447,9 @@ bool trsv_check_timeout(TrsvPlan &T, hipStream_t s) {
   LA_HIP(hipMemcpyAsync(&flag, T.err_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
   LA_HIP(hipStreamSynchronize(s));
   if (flag) LA_HIP(hipMemsetAsync(T.err_flag.p, 0, sizeof(int), s));
-  return flag != 0;
+  bool bad = flag != 0;
+  if (T.transposed) bad = trsv_check_timeout(*T.transposed, s) || bad;
+  return bad;
 }
 
 // ------------------------------------------------------------------ ILU / IC

@@ -4,10 +4,12 @@ diffusion operator (the C5 workload): loss ||x - x*||^2 with x* = A(theta*)^{-1}
 dL/dtheta and dL/db come from the torch.autograd.Function test binding
 (linearalgebra_amd/autograd.py) over the C ABI: a transposed IC(0)-PCG
 adjoint solve and the value-gradient kernel. Checked against
-  * an FP64 CPU reference gradient (oracle PCG solves + the closed form
-    dL/dtheta_e = -(lambda_i - lambda_j)(x_i - x_j)),  rel. error <= 1e-6
-    (both sides solve to a 1e-12 true relative residual, the attainable
-    floor for this operator in FP64);
+  * an FP64 CPU reference gradient (sparse direct LU solves, scipy's
+    SuperLU, + the closed form dL/dtheta_e = -(lambda_i - lambda_j)(x_i - x_j)),
+    rel. error <= 1e-6; the GPU forward solves stop at a 1e-11 and the
+    adjoint solves at a 1e-10 true relative residual (the adjoint
+    right-hand side is smooth, and a smooth right-hand side's attainable
+    FP64 relative residual for this operator is about 1e-11);
   * central finite differences on 64 seeded theta entries, error <= 1e-4 of
     the largest checked component (h = 3e-4 theta_e, truncation O(h^2));
   * the mixed FP32 path against the FP64 path, rel. error <= 1e-6.
@@ -52,18 +54,18 @@ def _loss_and_grads(S, th, th_star, system, dev):
 
 
 def _reference_gradient(S, nx):
-    """FP64 CPU: oracle IC(0)-PCG solves and the closed form."""
+    """FP64 CPU: direct sparse LU solves and the closed form."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
     A, _, _, _ = la.gen.diffusion2d(nx, nx, S["theta"])
     As, _, _, _ = la.gen.diffusion2d(nx, nx, S["theta_star"])
-    Ao, Aso = as_orc(A), as_orc(As)
-    def solve(M, rhs):
-        R, ok = orc.icc(orc.split_triangle(M, True), 0)
-        x, conv, it, rr = orc.pcg(M, rhs, kind="ic", F=R, rel_tol=1e-12, maxiter=20000)
-        assert conv
-        return x
-    x = solve(Ao, S["b"])
-    xs = solve(Aso, S["b"])
-    lam = solve(Ao, 2 * (x - xs))  # A symmetric: A^T = A
+    def lu(M):
+        return spl.splu(sp.csr_matrix((M.av, M.aj, M.ai), shape=(M.n, M.n)).tocsc(),
+                        permc_spec="MMD_AT_PLUS_A")
+    LA = lu(A)
+    x = LA.solve(S["b"])
+    xs = lu(As).solve(S["b"])
+    lam = LA.solve(2 * (x - xs))  # A symmetric: A^T = A
     g = -(lam[S["ei"]] - lam[S["ej"]]) * (x[S["ei"]] - x[S["ej"]])
     return g, lam, x
 
@@ -75,7 +77,8 @@ def as_orc(A):
 def test_adjoint_vs_fp64_reference_full_size(dev):
     nx = 512
     S = _setup(nx, dev)
-    system = S["SparseSystem"](S["A0"], kind="ic", level=0, rel_tol=1e-12, max_iters=20000)
+    system = S["SparseSystem"](S["A0"], kind="ic", level=0, rel_tol=1e-11, max_iters=20000,
+                               adjoint_rel_tol=1e-10)
     print("C5 512x512 adjoint", flush=True)
     L, g, gb, x = _loss_and_grads(S, S["theta"], S["theta_star"], system, dev)
     print(system.last, flush=True)
@@ -88,7 +91,8 @@ def test_adjoint_vs_fp64_reference_full_size(dev):
 def test_adjoint_finite_differences(dev):
     nx = 128
     S = _setup(nx, dev, seed=9)
-    system = S["SparseSystem"](S["A0"], kind="ic", level=0, rel_tol=1e-12, max_iters=20000)
+    system = S["SparseSystem"](S["A0"], kind="ic", level=0, rel_tol=1e-11, max_iters=20000,
+                               adjoint_rel_tol=1e-10)
     L0, g, gb, _ = _loss_and_grads(S, S["theta"], S["theta_star"], system, dev)
     rng = np.random.default_rng(17)
     idx = rng.choice(len(S["theta"]), size=64, replace=False)
@@ -123,9 +127,9 @@ def _loss_only(S, th, system, dev):
 def test_adjoint_mixed_precision_matches_fp64(dev):
     nx = 192
     S = _setup(nx, dev, seed=3)
-    sys64 = S["SparseSystem"](S["A0"], kind="ic", level=0, rel_tol=1e-12, max_iters=20000)
-    sys32 = S["SparseSystem"](S["A0"], kind="ic", level=0, rel_tol=1e-12, max_iters=20000,
-                              precision=la.PREC_MIXED_F32, inner_tol=1e-5)
+    kw = dict(kind="ic", level=0, rel_tol=1e-11, max_iters=20000, adjoint_rel_tol=1e-10)
+    sys64 = S["SparseSystem"](S["A0"], **kw)
+    sys32 = S["SparseSystem"](S["A0"], precision=la.PREC_MIXED_F32, inner_tol=1e-5, **kw)
     _, g64, gb64, _ = _loss_and_grads(S, S["theta"], S["theta_star"], sys64, dev)
     _, g32, gb32, _ = _loss_and_grads(S, S["theta"], S["theta_star"], sys32, dev)
     print(sys32.last, flush=True)
