@@ -19,7 +19,11 @@ int32 indices, RCM-reordered on the host.
     bounded ~10 s sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4]
-       [--no-pcg] [--no-cpu] [--pmc-probe]
+       [--no-pcg] [--no-cpu] [--pmc-probe] [--trsv level|syncfree] [--c4-grid G]
+
+--config c3 (power-law SpMV in FP64/FP32/BF16) and --config c4 (27-point
+160^3: SpMV, ILU(1)-GMRES(30) FP64 vs mixed precision) print their own
+JSON records; the default (c2) prints the bench line.
 """
 import argparse
 import json
@@ -141,6 +145,116 @@ def run_pcg(la, torch, A, dev, stream, trsv_mode, variants):
     return out
 
 
+def _spmv_table(la, torch, A, mats, dev, stream, algs, dtypes, steps, seed=77):
+    """GB/s and relative error against the FP64 oracle for each (alg, dtype)."""
+    from oracle import oracle as orc
+    n, nnz = A.n, A.nnz
+    x64 = torch.tensor(la.gen.uniform(n, seed), device=dev)
+    ref = orc.spmv(orc.CSR(n, A.ai, A.aj, A.av), x64.cpu().numpy(), alg="parallel",
+                   nthreads=orc.max_threads())
+    out = {}
+    for dt, name in dtypes:
+        for m in mats:
+            if dt != la.F64:
+                m.add_dtype(dt)
+        vec = torch.float64 if dt == la.F64 else torch.float32
+        acc = la.F64 if dt == la.F64 else la.F32
+        x = x64.to(vec)
+        y = torch.zeros(n, dtype=vec, device=dev)
+        for aname, alg in algs:
+            mats[0].spmv(x, y, alg=alg, val=dt, acc=acc)
+            torch.cuda.synchronize()
+            err = float(np.linalg.norm(y.double().cpu().numpy() - ref) / np.linalg.norm(ref))
+            d, _ = time_spmv(la, torch, mats, x, y, steps, 3, alg, dt, acc, stream)
+            sv = {la.F64: 8, la.F32: 4, la.BF16: 2}[dt]
+            sx = 8 if dt == la.F64 else 4
+            B = spmv_bytes(n, nnz, s_v=sv, s_x=sx, s_y=sx)
+            us = float(np.mean(d)) * 1e3
+            out["%s_%s" % (aname, name)] = {"us": round(us, 2), "GBs": round(B / us / 1e3, 1),
+                                            "rel_err_vs_fp64": float("%.3e" % err)}
+            log(aname, name, out["%s_%s" % (aname, name)])
+    return out
+
+
+def bench_c3(la, torch, dev, stream, args):
+    """Power-law matrix, n = 4M, lognormal row lengths (mean 16, sigma 1,
+    clipped to 20000), uniform deduplicated columns: SpMV load balance."""
+    t0 = time.time()
+    A = la.gen.powerlaw(4_000_000, 16.0, 1.0, 20000, seed=3)
+    prep = round(time.time() - t0, 3)
+    rl = np.diff(A.ai)
+    mats = [la.DeviceMatrix(A, stream=stream) for _ in range(2)]
+    tab = _spmv_table(la, torch, A, mats, dev, stream,
+                      [("albus", la.SPMV_ALBUS), ("row", la.SPMV_ROW)],
+                      [(la.F64, "f64"), (la.F32, "f32"), (la.BF16, "bf16")], args.steps)
+    print(json.dumps({"config": "c3 power-law n=4M", "n": A.n, "nnz": A.nnz,
+                      "max_row": int(rl.max()), "gen_s": prep, "spmv": tab,
+                      "note": "f32/bf16: values in that type, FP32 vectors and accumulation"}))
+
+
+def bench_c4(la, torch, dev, stream, args):
+    """27-point anisotropic variable-coefficient diffusion, 160^3: SpMV in
+    FP64/FP32/BF16, then ILU(1)-GMRES(30) to 1e-10, plain FP64 against
+    mixed precision with FP64 iterative refinement."""
+    grid = args.c4_grid
+    t0 = time.time()
+    A = la.gen.aniso27(grid)
+    t_gen = time.time() - t0
+    mats = [la.DeviceMatrix(A, stream=stream)]
+    tab = _spmv_table(la, torch, A, mats, dev, stream, [("albus", la.SPMV_ALBUS)],
+                      [(la.F64, "f64"), (la.F32, "f32"), (la.BF16, "bf16")], 10)
+    t0 = time.time()
+    fai, faj, fu = la.iluk_symbolic(A, 1)
+    t_sym = time.time() - t0
+    t0 = time.time()
+    F = la.iluk_numeric(A, fai, faj, fu)
+    t_num = time.time() - t0
+    t0 = time.time()
+    P = la.Preconditioner(F, "ilu")
+    P.set_stream(stream)
+    t_an = time.time() - t0
+    b = torch.tensor(la.gen.uniform(A.n, 4), device=dev)
+    mode = la.TRSV_LEVEL if args.trsv == "level" else la.TRSV_SYNCFREE
+    runs = {}
+    variants = [("fp64", dict(precision=la.PREC_FP64)),
+                ("mixed_f32", dict(precision=la.PREC_MIXED_F32, inner_tol=1e-5)),
+                ("mixed_bf16", dict(precision=la.PREC_MIXED_BF16, inner_tol=1e-3))]
+    for name, kw in variants:
+        x = torch.zeros(A.n, dtype=torch.float64, device=dev)
+        la.fgmres(mats[0], P, b, x, rel_tol=1e-10, restart=30, max_iters=3000,
+                  trsv_mode=mode, **kw)  # untimed: builds precision copies
+        x.zero_()
+        torch.cuda.synchronize()
+        r = la.fgmres(mats[0], P, b, x, rel_tol=1e-10, restart=30, max_iters=3000,
+                      trsv_mode=mode, **kw)
+        runs[name] = {"seconds": round(r["seconds"], 4), "iters": r["iters"],
+                      "outer": r["outer_iters"], "rel_residual": float("%.3e" % r["rel_residual"]),
+                      "converged": r["converged"]}
+        log("gmres", name, runs[name])
+    ap = {}
+    r = torch.tensor(la.gen.uniform(A.n, 5), device=dev)
+    z = torch.zeros_like(r)
+    for val, vn in [(la.F64, "f64"), (la.F32, "f32")]:
+        rr = r if val == la.F64 else r.float()
+        zz = z if val == la.F64 else z.float()
+        P.apply(rr, zz, mode=mode, val=val)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            P.apply(rr, zz, mode=mode, val=val)
+        torch.cuda.synchronize()
+        ap[vn] = round((time.perf_counter() - t0) / 5 * 1e3, 3)
+    out = {"config": "c4 aniso27 %d^3" % grid, "n": A.n, "nnz": A.nnz, "spmv": tab,
+           "ilu1": {"nnz": int(len(faj)), "levels_L_U": P.levels(), "gen_s": round(t_gen, 2),
+                    "symbolic_s": round(t_sym, 2), "numeric_s": round(t_num, 2),
+                    "analysis_upload_s": round(t_an, 2), "apply_ms": ap},
+           "trsv_mode": args.trsv, "gmres30": runs}
+    f64 = runs["fp64"]["seconds"]
+    for k in ("mixed_f32", "mixed_bf16"):
+        out["speedup_%s_vs_fp64" % k] = round(f64 / runs[k]["seconds"], 3)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +267,7 @@ def main():
                     help="only a few SpMV launches + a 1 GiB copy (for rocprofv3 --pmc)")
     ap.add_argument("--trsv", default="syncfree", choices=["level", "syncfree"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--c4-grid", type=int, default=160)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU sharding of the "
@@ -166,8 +281,12 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
 
+    if args.config == "c3":
+        return bench_c3(la, torch, dev, stream, args)
+    if args.config == "c4":
+        return bench_c4(la, torch, dev, stream, args)
     if args.config != "c2":
-        raise SystemExit("only --config c2 is a bench line; see tests for c1,c3,c4,c5")
+        raise SystemExit("--config is one of c2 (the bench line), c3, c4")
     A, prep = build_c2(la)
     n, nnz = A.n, A.nnz
     log("matrix n=%d nnz=%d prep=%s" % (n, nnz, prep))

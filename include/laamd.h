@@ -269,7 +269,8 @@ typedef struct {
   int precision;       /* la_precision                                     */
   int trsv_mode;       /* la_trsv_mode for the preconditioner apply (default SYNCFREE) */
   int spmv_alg;        /* la_spmv_alg                                      */
-  double inner_tol;    /* mixed: inner relative reduction per refinement   */
+  double inner_tol;    /* mixed: inner relative reduction per refinement (the deepest asked; a
+                          step stops earlier once that reaches rel_tol/abs_tol, margin 4) */
   int max_outer;       /* mixed: refinement steps                          */
   int print_level;     /* set_print_level: 1 prints each residual          */
 } la_solver_opts;

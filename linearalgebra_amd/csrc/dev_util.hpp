@@ -94,6 +94,15 @@ __device__ __forceinline__ V from_bits(typename Sentinel<V>::U u) {
 // ---------------------------------------------------------------- XCD id
 // MI355X has 8 XCDs, each with its own L2; workgroups are dealt to them
 // round-robin in dispatch order.
+// Block b of a launch of nb blocks, renumbered so that each XCD gets one
+// contiguous range of logical blocks (a bijection on [0, nb) whatever the
+// placement; with round-robin placement neighbouring work, and the vector
+// lines it gathers, stays in one XCD's L2).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned xcd = b & 7u, idx = b >> 3, per = nb >> 3, rem = nb & 7u;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
 // the XCD this wavefront runs on (HW_REG_XCC_ID, bits 3:0)
 __device__ __forceinline__ unsigned xcc_id() {
   unsigned v;

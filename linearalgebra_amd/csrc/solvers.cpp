@@ -261,13 +261,18 @@ int run_solver(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double
       if (total >= o.max_iters) break;
       copy_vec(n, LA_F64, r64, LA_F32, rhs, 1.0 / rn, s);
       fill_zero(n, LA_F32, d32.p, s);
-      CgResult R = core(vd, LA_F32, static_cast<const void *>(rhs), d32.p, 1.0, o.inner_tol, 0.0,
+      // the inner solve needs no more than the reduction that brings the
+      // outer residual to the target (with a margin of 4 for the low-
+      // precision operator); at most inner_tol, at least a factor 10
+      const double goal = o.abs_tol > 0 ? std::max(o.rel_tol * nb, o.abs_tol) : o.rel_tol * nb;
+      const double inner = std::min(0.1, std::max(o.inner_tol, 0.25 * goal / rn));
+      CgResult R = core(vd, LA_F32, static_cast<const void *>(rhs), d32.p, 1.0, inner, 0.0,
                         o.max_iters - total, W);
       total += R.iters;
       add_scaled(n, LA_F32, d32.p, rn, x, s);
     }
     out.iters = total;
-    out.outer_iters = outer + (conv ? 0 : 0);
+    out.outer_iters = outer;
     out.converged = conv;
   }
   LA_HIP(hipStreamSynchronize(s));

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_seg(
   static_assert(kSegE == 8, "two quads per lane");
   __shared__ A sp[kBlock / 64][kSegNnz];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t s = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + wave;
+  const int64_t s = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * (kBlock / 64) + wave;
   const bool live = s < nseg;
   const int64_t g0 = s * kSegNnz;
   const int64_t g1 = live ? (g0 + kSegNnz < nnz ? g0 + kSegNnz : nnz) : g0;

@@ -60,8 +60,8 @@ __global__ __launch_bounds__(kB) void k_trsv_level(int k0, int k1, const int *__
   }
 }
 
-// x <- sentinel everywhere; ctl[0] (ticket) and ctl[1] (owning XCD + 1)
-// reset for the next sync-free solve.
+// x <- sentinel everywhere; the control words ctl[0..15] (ctl[0]: ticket,
+// ctl[1]: owning XCD + 1) reset for the next sync-free solve.
 template <typename V>
 __global__ void k_fill_sentinel(int64_t n, V *x, unsigned *ctl) {
   using U = typename Sentinel<V>::U;
@@ -69,7 +69,7 @@ __global__ void k_fill_sentinel(int64_t n, V *x, unsigned *ctl) {
   for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
        t += static_cast<int64_t>(gridDim.x) * kB)
     xb[t] = Sentinel<V>::bits;
-  if (blockIdx.x == 0 && threadIdx.x < 2) ctl[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 16) ctl[threadIdx.x] = 0;
 }
 
 // Sync-free solve. Readiness is polled with agent-scope relaxed loads (they
@@ -90,12 +90,67 @@ __global__ void k_fill_sentinel(int64_t n, V *x, unsigned *ctl) {
 // every 256 polls and gives up too, so the kernel always drains quickly.
 constexpr uint64_t kWaitTicks = 20000000;  // 0.2 s
 
+// One row of a sync-free solve, G lanes: wait for each x_j, accumulate,
+// publish x_r. `sub` is the lane's index inside its group.
+template <int G, typename T, typename V, typename A, bool DIAG, bool XLOCAL>
+__device__ __forceinline__ void sf_row(int k, int sub, const int *__restrict__ rowptr,
+                                       const int *__restrict__ col, const T *__restrict__ val,
+                                       const T *__restrict__ diag, const int *__restrict__ perm,
+                                       const V *__restrict__ b, V *x, int *err) {
+  using U = typename Sentinel<V>::U;
+  // everything that does not depend on x first
+  const int rs = rowptr[k], re = rowptr[k + 1];
+  const int r = perm[k];
+  const A bb = A(b[r]);
+  A dd = A(1);
+  if constexpr (DIAG) dd = A(ldt(diag, k));
+  A acc = 0;
+  bool gave_up = false;
+  // one entry per lane at a time: a lane that loads only entries of its own
+  // row issues no extra request (the polls below go to the memory side, and
+  // their count is what the solve pays for)
+  for (int q = rs + sub; q < re; q += G) {
+    const int j = ldi(col, q);
+    const A a = A(ldt(val, q));
+    V xv = load_ready(x + j);
+    if (is_sentinel(xv) && !gave_up) {
+      const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+      unsigned polls = 0;
+      while (is_sentinel(xv)) {
+        __builtin_amdgcn_s_sleep(1);
+        xv = load_ready(x + j);
+        if ((++polls & 255u) == 0) {
+          if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+              __builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+            gave_up = true;
+            break;
+          }
+        }
+      }
+    }
+    acc += a * A(is_sentinel(xv) ? V(0) : xv);
+  }
+  if (gave_up) __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  acc = group_sum<G>(acc);
+  if (sub == 0) {
+    A v = bb - acc;
+    if constexpr (DIAG) v = v / dd;
+    U o = to_bits<V>(V(v));
+    if (o == Sentinel<V>::bits) o = Sentinel<V>::qnan;
+    if constexpr (XLOCAL)
+      __hip_atomic_store(reinterpret_cast<U *>(x + r), o, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+      __hip_atomic_store(reinterpret_cast<U *>(x + r), o, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 template <int G, int NT, typename T, typename V, typename A, bool DIAG, bool XLOCAL>
 __global__ __launch_bounds__(NT) void k_trsv_syncfree(
     int nchunks, const int *__restrict__ chunk_start, const int *__restrict__ rowptr,
     const int *__restrict__ col, const T *__restrict__ val, const T *__restrict__ diag,
     const int *__restrict__ perm, const V *__restrict__ b, V *x, unsigned *ctl, int *err) {
-  using U = typename Sentinel<V>::U;
   if constexpr (XLOCAL) {
     __shared__ int s_go;
     if (threadIdx.x == 0) {
@@ -108,62 +163,23 @@ __global__ __launch_bounds__(NT) void k_trsv_syncfree(
   }
   // row of this lane group inside a chunk of NT/G rows
   const int gi = static_cast<int>(threadIdx.x) / G, sub = threadIdx.x & (G - 1);
-  // two ticket slots: thread 0 writes the next one while slow waves may
-  // still read the current one; one barrier per chunk suffices
+  // two ticket slots: while a chunk is solved, thread 0 already claims the
+  // next ticket into the other slot, so the claim's round trip overlaps the
+  // work; one barrier per chunk. Holding one ticket ahead keeps the drain
+  // argument: the lowest unfinished chunk is either some workgroup's
+  // current chunk or the next one of a workgroup whose current chunk is
+  // finished.
   __shared__ int s_c[2];
+  if (threadIdx.x == 0) s_c[0] = static_cast<int>(atomicAdd(&ctl[0], 1u));
+  __syncthreads();
   for (int it = 0;; ++it) {
-    if (threadIdx.x == 0) s_c[it & 1] = static_cast<int>(atomicAdd(&ctl[0], 1u));
-    __syncthreads();
     const int c = s_c[it & 1];
     if (c >= nchunks) break;
-    const int k0 = chunk_start[c], k1 = chunk_start[c + 1];
-    const int k = k0 + gi;
-    if (k < k1) {
-      // everything that does not depend on x first
-      const int rs = rowptr[k], re = rowptr[k + 1];
-      const int r = perm[k];
-      const A bb = A(b[r]);
-      A dd = A(1);
-      if constexpr (DIAG) dd = A(ldt(diag, k));
-      A acc = 0;
-      bool gave_up = false;
-      for (int q = rs + sub; q < re; q += G) {
-        const int j = ldi(col, q);
-        const A a = A(ldt(val, q));
-        V xv = load_ready(x + j);
-        if (is_sentinel(xv) && !gave_up) {
-          const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-          unsigned polls = 0;
-          while (is_sentinel(xv)) {
-            __builtin_amdgcn_s_sleep(1);
-            xv = load_ready(x + j);
-            if ((++polls & 255u) == 0) {
-              if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-                  __builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
-                gave_up = true;
-                break;
-              }
-            }
-          }
-        }
-        acc += a * A(is_sentinel(xv) ? V(0) : xv);
-      }
-      if (gave_up)
-        __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      acc = group_sum<G>(acc);
-      if (sub == 0) {
-        A v = bb - acc;
-        if constexpr (DIAG) v = v / dd;
-        U o = to_bits<V>(V(v));
-        if (o == Sentinel<V>::bits) o = Sentinel<V>::qnan;
-        if constexpr (XLOCAL)
-          __hip_atomic_store(reinterpret_cast<U *>(x + r), o, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_WORKGROUP);
-        else
-          __hip_atomic_store(reinterpret_cast<U *>(x + r), o, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
+    if (threadIdx.x == 0) s_c[(it + 1) & 1] = static_cast<int>(atomicAdd(&ctl[0], 1u));
+    const int k = chunk_start[c] + gi;
+    if (k < chunk_start[c + 1])
+      sf_row<G, T, V, A, DIAG, XLOCAL>(k, sub, rowptr, col, val, diag, perm, b, x, err);
+    __syncthreads();
   }
 }
 
@@ -198,6 +214,19 @@ void launch_level_range(const TrsvPlan &P, int l0, int l1, const T *val, const T
   LA_HIP(hipGetLastError());
 }
 
+template <int G, int NT, bool XL, typename T, typename V, typename A, bool DIAG>
+void sf_launch(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x, hipStream_t s) {
+  int *rp = P.rowptr.as<int>(), *cl = P.col.as<int>(), *pm = P.perm.as<int>();
+  unsigned *ctl = P.counter.as<unsigned>();
+  int *err = P.err_flag.as<int>();
+  // XLOCAL launches 8x the workgroups it needs (the others exit at once);
+  // otherwise the whole chip, `sf_wgs` workgroups in total
+  const unsigned need = std::max(1u, static_cast<unsigned>(P.nchunks));
+  const unsigned grid = XL ? 8u * std::min(need, P.sf_wgs) : std::min(need, P.sf_wgs);
+  hipLaunchKernelGGL((k_trsv_syncfree<G, NT, T, V, A, DIAG, XL>), dim3(grid), dim3(NT), 0, s,
+                     P.nchunks, P.chunk_start.as<int>(), rp, cl, val, diag, pm, b, x, ctl, err);
+}
+
 template <typename T, typename V, typename A, bool DIAG>
 void launch_syncfree(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x, hipStream_t s) {
   const int64_t n = P.n;
@@ -205,29 +234,19 @@ void launch_syncfree(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x,
   if (fg == 0) fg = 1;
   hipLaunchKernelGGL(k_fill_sentinel<V>, dim3(fg), dim3(kB), 0, s, n, x, P.counter.as<unsigned>());
   LA_HIP(hipGetLastError());
-  // XLOCAL: one XCD does the solve: 8x the workgroups it needs (the others
-  // exit at once); otherwise the whole chip, `sf_wgs` workgroups in total
-  const unsigned need = std::max(1u, static_cast<unsigned>(P.nchunks));
-  const unsigned grid = P.sf_xlocal ? 8u * std::min<unsigned>(need, P.sf_wgs)
-                                    : std::min<unsigned>(need, P.sf_wgs);
-#define LA_SF3(GG, NT, XL)                                                                   \
-  hipLaunchKernelGGL((k_trsv_syncfree<GG, NT, T, V, A, DIAG, XL>), dim3(grid), dim3(NT), 0, s, \
-                     P.nchunks, P.chunk_start.as<int>(), P.rowptr.as<int>(), P.col.as<int>(),  \
-                     val, diag, P.perm.as<int>(), b, x, P.counter.as<unsigned>(),             \
-                     P.err_flag.as<int>())
-#define LA_SF2(GG, NT)        \
-  if (P.sf_xlocal) {          \
-    LA_SF3(GG, NT, true);     \
-  } else {                    \
-    LA_SF3(GG, NT, false);    \
+#define LA_SF2(GG, NT)                                      \
+  if (P.sf_xlocal) {                                        \
+    sf_launch<GG, NT, true, T, V, A, DIAG>(P, val, diag, b, x, s);  \
+  } else {                                                  \
+    sf_launch<GG, NT, false, T, V, A, DIAG>(P, val, diag, b, x, s); \
   }
-#define LA_SF(GG)                          \
-  if (P.sf_nt == 1024) {                   \
-    LA_SF2(GG, 1024);                      \
-  } else if (P.sf_nt == 512) {             \
-    LA_SF2(GG, 512);                       \
-  } else {                                 \
-    LA_SF2(GG, 256);                       \
+#define LA_SF(GG)              \
+  if (P.sf_nt == 1024) {       \
+    LA_SF2(GG, 1024);          \
+  } else if (P.sf_nt == 512) { \
+    LA_SF2(GG, 512);           \
+  } else {                     \
+    LA_SF2(GG, 256);           \
   }
   switch (P.G) {
     case 1: LA_SF(1); break;
@@ -238,7 +257,6 @@ void launch_syncfree(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x,
     case 32: LA_SF(32); break;
     default: LA_SF(64); break;
   }
-#undef LA_SF3
 #undef LA_SF2
 #undef LA_SF
   LA_HIP(hipGetLastError());
@@ -330,6 +348,7 @@ std::unique_ptr<TrsvPlan> make_trsv(int n, int base, const int *ai, const int *a
     if (diag) dg[k] = diag[r];
   }
   P->G = pick_G(n ? static_cast<double>(nnz) / n : 1.0);
+
   // sync-free launch shape. Narrow levels (few rows per level): the solve is
   // a chain of dependency hops, so keep it on one XCD, where a hop costs
   // ~0.3 us instead of ~0.55 us, with 32 workgroups of 256. Wide levels:
@@ -343,9 +362,12 @@ std::unique_ptr<TrsvPlan> make_trsv(int n, int base, const int *ai, const int *a
     P->sf_wgs = 32;
     P->sf_nt = 256;
   } else {
+    // wide levels: the whole chip, 256 workgroups; rows of 16+ lanes get
+    // 1024-thread workgroups, so a level costs half the tickets (27-point
+    // ILU(1) at 160^3: 5.4 ms per apply against 6.9 ms with 512)
     P->sf_xlocal = false;
     P->sf_wgs = 256;
-    P->sf_nt = 512;
+    P->sf_nt = P->G >= 16 ? 1024 : 512;
   }
   // chunks: one workgroup's worth of rows (sf_nt/G), never spanning a level
   if (const char *e = std::getenv("LAAMD_SF_NT")) {
@@ -366,14 +388,14 @@ std::unique_ptr<TrsvPlan> make_trsv(int n, int base, const int *ai, const int *a
   P->col.alloc(sizeof(int) * static_cast<size_t>(nnz));
   P->perm.alloc(sizeof(int) * static_cast<size_t>(std::max(n, 1)));
   P->chunk_start.alloc(sizeof(int) * chunks.size());
-  P->counter.alloc(sizeof(unsigned) * 4);
+  P->counter.alloc(sizeof(unsigned) * 16);
   P->err_flag.alloc(sizeof(int) * 4);
   P->rowptr.upload(rp.data(), sizeof(int) * (n + 1), s);
   P->col.upload(cl.data(), sizeof(int) * nnz, s);
   P->perm.upload(iperm.data(), sizeof(int) * n, s);
   P->chunk_start.upload(chunks.data(), sizeof(int) * chunks.size(), s);
   LA_HIP(hipMemsetAsync(P->err_flag.p, 0, sizeof(int) * 4, s));
-  LA_HIP(hipMemsetAsync(P->counter.p, 0, sizeof(unsigned) * 4, s));
+  LA_HIP(hipMemsetAsync(P->counter.p, 0, sizeof(unsigned) * 16, s));
   P->val.set_from_host(vl.data(), nnz, s);
   if (diag) P->diag.set_from_host(dg.data(), n, s);
   LA_HIP(hipStreamSynchronize(s));
