@@ -22,7 +22,7 @@ struct Mat {
   // nnz-balanced segment table (ALBUS / SEGSUM)
   bool seg_ready = false;
   int64_t nseg = 0;
-  DevBuf seg_first_row, seg_head, seg_tail, seg_tail_row;
+  DevBuf seg_first_row, seg_head, seg_tail, seg_tail_row, seg_tail_end;
   // A^T, built on first transposed product
   std::unique_ptr<Mat> transposed;
   // host copy of the pattern, kept for the transpose and the value map

@@ -129,7 +129,8 @@ template <typename T, typename V, typename A>
 __global__ __launch_bounds__(kBlock) void k_spmv_seg(
     int n, int64_t nnz, int64_t nseg, const int *__restrict__ ai, const int *__restrict__ aj,
     const T *__restrict__ av, const V *__restrict__ x, const int *__restrict__ first_row,
-    A *head_part, A *tail_part, int *tail_row, V *y, const V *z, A alpha, A beta) {
+    A *head_part, A *tail_part, int *tail_row, int *tail_end, V *y, const V *z, A alpha,
+    A beta) {
   using Q = typename Quad<T>::type;
   static_assert(kSegE == 8, "two quads per lane");
   __shared__ A sp[kBlock / 64][kSegNnz];
@@ -182,7 +183,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_seg(
     acc = wave_sum(acc);
     if (lane == 0) head_part[s] = acc;
   }
-  if (lane == 0) tail_row[s] = (f1 > f0 && static_cast<int64_t>(ai[f1]) > g1) ? f1 - 1 : -1;
+  if (lane == 0) {
+    const int e = ai[f1];
+    const bool crosses = f1 > f0 && static_cast<int64_t>(e) > g1;
+    tail_row[s] = crosses ? f1 - 1 : -1;
+    tail_end[s] = e;
+  }
   int chunk = 0;
   for (int rb = f0; rb < f1; rb += 64, ++chunk) {
     const int R = (f1 - rb) < 64 ? (f1 - rb) : 64;
@@ -203,21 +209,21 @@ __global__ __launch_bounds__(kBlock) void k_spmv_seg(
 }
 
 // rows that cross segment boundaries: tail partial + the following head
-// partials, added in segment order
+// partials, added in segment order. The row's end comes from the segment
+// kernel, so every load but the rare third head goes out at once.
 template <typename V, typename A>
-__global__ void k_spmv_seg_fix(int64_t nseg, const int *__restrict__ ai,
-                               const int *__restrict__ tail_row, const A *head_part,
+__global__ void k_spmv_seg_fix(int64_t nseg, const int *__restrict__ tail_row,
+                               const int *__restrict__ tail_end, const A *head_part,
                                const A *tail_part, V *y, const V *z, A alpha, A beta) {
   const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= nseg) return;
   const int r = tail_row[s];
+  const int64_t end = tail_end[s];
+  const A tp = tail_part[s];
+  const A hp = s + 1 < nseg ? head_part[s + 1] : A(0);
   if (r < 0) return;
-  A acc = tail_part[s];
-  const int64_t end = ai[r + 1];
-  for (int64_t t = s + 1; t < nseg; ++t) {
-    acc += head_part[t];
-    if (end <= (t + 1) * kSegNnz) break;
-  }
+  A acc = tp + hp;
+  for (int64_t t = s + 2; t < nseg && end > t * kSegNnz; ++t) acc += head_part[t];
   A o = alpha * acc;
   if (beta != A(0)) o += beta * A(z[r]);
   y[r] = V(o);
@@ -267,10 +273,12 @@ void launch_seg(Mat &M, const T *av, const V *x, V *y, const V *z, A alpha, A be
   A *head = static_cast<A *>(M.seg_head.p), *tail = static_cast<A *>(M.seg_tail.p);
   hipLaunchKernelGGL((k_spmv_seg<T, V, A>), dim3(grid), dim3(kBlock), 0, s, M.rows, M.nnz,
                      nseg, M.ai.as<int>(), M.aj.as<int>(), av, x, M.seg_first_row.as<int>(),
-                     head, tail, M.seg_tail_row.as<int>(), y, z, alpha, beta);
+                     head, tail, M.seg_tail_row.as<int>(), M.seg_tail_end.as<int>(), y, z,
+                     alpha, beta);
   LA_HIP(hipGetLastError());
   hipLaunchKernelGGL((k_spmv_seg_fix<V, A>), dim3((nseg + 255) / 256), dim3(256), 0, s, nseg,
-                     M.ai.as<int>(), M.seg_tail_row.as<int>(), head, tail, y, z, alpha, beta);
+                     M.seg_tail_row.as<int>(), M.seg_tail_end.as<int>(), head, tail, y, z,
+                     alpha, beta);
   LA_HIP(hipGetLastError());
 }
 
@@ -338,6 +346,7 @@ void spmv_prepare_segments(Mat &A) {
   A.seg_head.alloc(sizeof(double) * nseg);
   A.seg_tail.alloc(sizeof(double) * nseg);
   A.seg_tail_row.alloc(sizeof(int) * nseg);
+  A.seg_tail_end.alloc(sizeof(int) * nseg);
   hipLaunchKernelGGL(k_seg_table, dim3((nseg + 1 + 255) / 256), dim3(256), 0, A.stream, A.rows,
                      nseg, A.ai.as<int>(), A.seg_first_row.as<int>());
   LA_HIP(hipGetLastError());
