@@ -91,7 +91,7 @@ def cpu_baseline(A, seconds=10.0):
         orc.spmv(Ao, x, alg="parallel", nthreads=threads)
         reps += 1
         el = time.perf_counter() - t0
-        if el >= seconds or reps >= 2000:
+        if el >= seconds or reps >= 100000:
             break
     per = el / reps
     B = spmv_bytes(A.n, A.nnz)
