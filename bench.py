@@ -216,8 +216,12 @@ def bench_c4(la, torch, dev, stream, args):
     b = torch.tensor(la.gen.uniform(A.n, 4), device=dev)
     mode = la.TRSV_LEVEL if args.trsv == "level" else la.TRSV_SYNCFREE
     runs = {}
+    # the configuration's two mixed forms: FP32 factors with the matrix in
+    # FP32 or BF16; plus everything in BF16
     variants = [("fp64", dict(precision=la.PREC_FP64)),
                 ("mixed_f32", dict(precision=la.PREC_MIXED_F32, inner_tol=1e-5)),
+                ("mixed_bf16_matrix_f32_factors", dict(precision=la.PREC_MIXED_BF16_F32P,
+                                                       inner_tol=1e-3)),
                 ("mixed_bf16", dict(precision=la.PREC_MIXED_BF16, inner_tol=1e-3))]
     for name, kw in variants:
         x = torch.zeros(A.n, dtype=torch.float64, device=dev)
@@ -250,7 +254,7 @@ def bench_c4(la, torch, dev, stream, args):
                     "analysis_upload_s": round(t_an, 2), "apply_ms": ap},
            "trsv_mode": args.trsv, "gmres30": runs}
     f64 = runs["fp64"]["seconds"]
-    for k in ("mixed_f32", "mixed_bf16"):
+    for k in ("mixed_f32", "mixed_bf16_matrix_f32_factors", "mixed_bf16"):
         out["speedup_%s_vs_fp64" % k] = round(f64 / runs[k]["seconds"], 3)
     print(json.dumps(out))
 

@@ -15,7 +15,7 @@ F64, F32, BF16 = 0, 1, 2
 SPMV_ROW, SPMV_SEGSUM, SPMV_ALBUS, SPMV_AUTO = 0, 1, 2, 3
 TRSV_LEVEL, TRSV_SYNCFREE = 0, 1
 LOWER, UPPER = 0, 1
-PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16 = 0, 1, 2
+PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16, PREC_MIXED_BF16_F32P = 0, 1, 2, 3
 STOP_TRUE_RESIDUAL, STOP_RECURRENCE = 0, 1
 
 

@@ -57,7 +57,12 @@ double norm2(int64_t n, la_dtype dt, const void *x, Scratch &S, hipStream_t s) {
   return std::sqrt(fetch(S, d, s));
 }
 
+// value types of the inner solve: the matrix, and the preconditioner factors
 la_dtype inner_val_dtype(int precision) {
+  return precision == LA_PREC_MIXED_BF16 || precision == LA_PREC_MIXED_BF16_F32P ? LA_BF16
+                                                                                 : LA_F32;
+}
+la_dtype inner_prec_dtype(int precision) {
   return precision == LA_PREC_MIXED_BF16 ? LA_BF16 : LA_F32;
 }
 
@@ -66,6 +71,7 @@ thread_local bool tl_transpose = false;
 void apply_M(Prec *P, const la_solver_opts &o, la_dtype val, la_dtype vec, const void *r,
              void *z, int64_t n, hipStream_t s) {
   const la_trsv_mode m = static_cast<la_trsv_mode>(o.trsv_mode);
+  if (val != LA_F64) val = inner_prec_dtype(o.precision);
   if (!P) copy_vec(n, vec, r, vec, z, 1.0, s);
   else if (tl_transpose) prec_apply_transpose(*P, m, val, vec, r, z, s);
   else prec_apply(*P, m, val, vec, r, z, s);
@@ -225,6 +231,8 @@ using clk = std::chrono::steady_clock;
 template <typename Core>
 int run_solver(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
                la_solver_result *res, Core core) {
+  LA_CHECK_ARG(o.precision >= LA_PREC_FP64 && o.precision <= LA_PREC_MIXED_BF16_F32P,
+               "unknown precision policy");
   hipStream_t s = A.stream;
   const int64_t n = A.rows;
   Work &W = work_for(n);
@@ -245,7 +253,7 @@ int run_solver(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double
   } else {
     const la_dtype vd = inner_val_dtype(o.precision);
     if (!A.vals.has[vd]) A.vals.add(vd, s);
-    if (P) prec_add_dtype(*P, vd);
+    if (P) prec_add_dtype(*P, inner_prec_dtype(o.precision));
     double *r64 = W.v64[5].as<double>();
     float *rhs = W.v32[5].as<float>();
     // the inner solution lives in a dedicated FP32 vector outside the ones

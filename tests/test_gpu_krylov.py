@@ -139,7 +139,8 @@ def test_fgmres_ilu(dev, mode):
     assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
-@pytest.mark.parametrize("prec", [la.PREC_MIXED_F32, la.PREC_MIXED_BF16])
+@pytest.mark.parametrize("prec", [la.PREC_MIXED_F32, la.PREC_MIXED_BF16,
+                                  la.PREC_MIXED_BF16_F32P])
 def test_fgmres_mixed_ir(dev, prec):
     import torch
     A = _o(la.gen.aniso27(14, 14, 14, aniso=100.0))
@@ -152,6 +153,17 @@ def test_fgmres_mixed_ir(dev, prec):
                     max_iters=3000, precision=prec, inner_tol=1e-4)
     print(res, flush=True)
     assert res["converged"] and res["rel_residual"] < 1e-10
+
+
+def test_unknown_precision_rejected(dev):
+    import torch
+    A = la.gen.poisson2d(8)
+    dA = la.DeviceMatrix(A)
+    b = torch.ones(A.n, dtype=torch.float64, device=dev)
+    x = torch.zeros_like(b)
+    with pytest.raises(la.LaError) as e:
+        la.pcg(dA, None, b, x, raise_on_fail=True, precision=9)
+    assert e.value.code == 1  # LA_ERR_ARG
 
 
 def test_pcg_ex5_reference_fixture(dev):
