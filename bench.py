@@ -113,7 +113,7 @@ def _cpu_model():
     return "unknown"
 
 
-def run_pcg(la, torch, A, dev, stream, trsv_mode, variants):
+def run_pcg(la, torch, A, dev, stream, trsv_mode, variants, cpu=False):
     U = la.split_triangle(A, upper=True)
     t0 = time.time()
     R, ok = la.icc(U, 0)
@@ -137,11 +137,29 @@ def run_pcg(la, torch, A, dev, stream, trsv_mode, variants):
         x.zero_()
         torch.cuda.synchronize()
         res = la.pcg(dA, P, b, x, rel_tol=1e-10, max_iters=5000, trsv_mode=trsv_mode, **kw)
+        if name == "fp64":
+            x_fp64 = x.cpu().numpy()
         out["runs"][name] = {"seconds": round(res["seconds"], 5), "iters": res["iters"],
                              "outer": res["outer_iters"],
                              "rel_residual": float("%.3e" % res["rel_residual"]),
                              "converged": res["converged"]}
         log("pcg", name, out["runs"][name])
+    if cpu:
+        # the same solve by the oracle's serial FP64 PCG (the reference's
+        # solver loop restated, one core), for comparison
+        from oracle import oracle as orc
+        Ao = orc.CSR(A.n, A.ai, A.aj, A.av)
+        Ro = orc.CSR(R.n, R.ai, R.aj, R.av)
+        bh = b.cpu().numpy()
+        t0 = time.perf_counter()
+        xo, conv, it, rr = orc.pcg(Ao, bh, kind="ic", F=Ro, rel_tol=1e-10, maxiter=5000)
+        el = time.perf_counter() - t0
+        out["cpu_fp64"] = {"seconds": round(el, 3), "iters": it, "rel_residual": float("%.3e" % rr),
+                           "converged": bool(conv), "cores": 1,
+                           "kind": "port (oracle PCG + IC(0) apply, serial)",
+                           "gpu_fp64_x_rel_diff": float("%.3e" % (
+                               np.linalg.norm(x_fp64 - xo) / np.linalg.norm(xo)))}
+        log("pcg cpu", out["cpu_fp64"])
     return out
 
 
@@ -355,7 +373,7 @@ def main():
                                              stop_rule=la.STOP_RECURRENCE)),
                     ("mixed_f32", dict(precision=la.PREC_MIXED_F32, inner_tol=1e-5)),
                     ("mixed_bf16", dict(precision=la.PREC_MIXED_BF16, inner_tol=1e-4))]
-        pcg = run_pcg(la, torch, A, dev, stream, mode, variants)
+        pcg = run_pcg(la, torch, A, dev, stream, mode, variants, cpu=not args.no_cpu)
         r = pcg["runs"]
         best64 = min(r["fp64"]["seconds"], r["fp64_recurrence"]["seconds"])
         pcg["speedup_mixed_f32_vs_fp64"] = round(best64 / r["mixed_f32"]["seconds"], 3)
