@@ -1,0 +1,121 @@
+"""Parity at the full benchmark sizes, through the C ABI.
+
+* 128^3 7-point Laplacian (RCM), n = 2,097,152: SpMV in every form against
+  the oracle's ParallelSPMV (bitwise equal to SerialSPMV), FP64 1e-12
+  element-wise of (|A||x|)_i; the IC(0) apply (both modes) against the
+  oracle's serial apply, 1e-12 relative; IC(0)-PCG to 1e-10 against the
+  oracle's run stored in tests/golden/c2_ic0_pcg.json (177 iterations):
+  iteration count within 2, true residual < 1e-10, x within 1e-8 relative
+  (both runs stop at ~9e-11 on the same Krylov path; cond(A) ~ 6.6e3).
+* Power-law matrix, n = 4,000,000, nnz ~ 64.0M: SpMV FP64 (1e-12 as
+  above), FP32 values (2e-6 of (|A||x|)_i) and BF16 values (2e-2), FP32
+  accumulation, against the FP64 oracle.
+* 27-point anisotropic 160^3, n = 4,096,000, nnz = 109,215,352: SpMV FP64
+  and the ILU(1) apply (both modes) against the oracle, 1e-12.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import linearalgebra_amd as la
+from helpers import GOLDEN, as_lib_csr, orc, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _o(A):
+    return orc.CSR(A.n, A.ai, A.aj, A.av, ncols=A.ncols)
+
+
+def _absbound(A, x):
+    return orc.spmv(orc.CSR(A.n, A.ai, A.aj, np.abs(A.av), ncols=A.ncols), np.abs(x),
+                    alg="parallel", nthreads=orc.max_threads())
+
+
+def _check_spmv(dev, A, algs, dtypes, seed=11):
+    import torch
+    d = la.DeviceMatrix(A)
+    x = la.gen.uniform(A.n, seed)
+    ref = orc.spmv(_o(A), x, alg="parallel", nthreads=orc.max_threads())
+    bound = _absbound(A, x)
+    for dt, tol in dtypes:
+        if dt != la.F64:
+            d.add_dtype(dt)
+        vec = torch.float64 if dt == la.F64 else torch.float32
+        xt = torch.tensor(x, device=dev).to(vec)
+        for alg in algs:
+            y = torch.zeros(A.n, dtype=vec, device=dev)
+            d.spmv(xt, y, alg=alg, val=dt, acc=la.F64 if dt == la.F64 else la.F32)
+            yh = y.double().cpu().numpy()
+            print("n=%d alg=%d dtype=%d max err/bound=%.3e" % (
+                A.n, alg, dt, np.max(np.abs(yh - ref) / np.maximum(bound, 1e-300))), flush=True)
+            assert np.all(np.abs(yh - ref) <= tol * bound + 1e-300), (alg, dt)
+    return d
+
+
+def _c2():
+    A = la.gen.laplace3d(128)
+    ip, p = la.rcm(A)
+    return la.permute(A, ip, p)
+
+
+def test_c2_spmv_full(dev):
+    A = _c2()
+    _check_spmv(dev, A, [la.SPMV_ROW, la.SPMV_SEGSUM, la.SPMV_ALBUS],
+                [(la.F64, 1e-12), (la.F32, 2e-6), (la.BF16, 2e-2)])
+
+
+def test_c2_ic0_apply_and_pcg_full(dev):
+    import torch
+    A = _c2()
+    U = la.split_triangle(A, upper=True)
+    R, ok = la.icc(U, 0)
+    assert ok
+    P = la.Preconditioner(R, "ic")
+    r = la.gen.uniform(A.n, 5)
+    zr = orc.prec_apply(_o(A), "ic", _o(R), r)
+    rt = torch.tensor(r, device=dev)
+    for mode in (la.TRSV_LEVEL, la.TRSV_SYNCFREE):
+        z = torch.zeros_like(rt)
+        P.apply(rt, z, mode=mode)
+        err = rel_err(z.cpu().numpy(), zr)
+        print("IC(0) apply mode=%d rel err %.3e" % (mode, err), flush=True)
+        assert err < 1e-12
+    gold = json.load(open(os.path.join(GOLDEN, "c2_ic0_pcg.json")))
+    b = torch.tensor(la.gen.uniform(A.n, 2024), device=dev)
+    x = torch.zeros_like(b)
+    res = la.pcg(la.DeviceMatrix(A), P, b, x, rel_tol=1e-10, max_iters=5000)
+    print(res, "oracle iters", gold["iters"], flush=True)
+    assert res["converged"] and res["rel_residual"] < 1e-10
+    assert abs(res["iters"] - gold["iters"]) <= 2
+    xh = x.cpu().numpy()
+    assert abs(np.linalg.norm(xh) - gold["x_norm"]) <= 1e-8 * gold["x_norm"]
+    xs = np.array(gold["sample_x"])
+    assert np.max(np.abs(xh[gold["sample_index"]] - xs)) <= 1e-8 * np.max(np.abs(xs))
+
+
+def test_c3_powerlaw_spmv_full(dev):
+    A = la.gen.powerlaw(4_000_000, 16.0, 1.0, 20000, seed=3)
+    assert A.n == 4_000_000 and 60e6 < A.nnz < 70e6
+    _check_spmv(dev, A, [la.SPMV_ALBUS, la.SPMV_ROW],
+                [(la.F64, 1e-12), (la.F32, 2e-6), (la.BF16, 2e-2)])
+
+
+def test_c4_spmv_and_ilu1_apply_full(dev):
+    import torch
+    A = la.gen.aniso27(160)
+    assert (A.n, A.nnz) == (4_096_000, 109_215_352)
+    _check_spmv(dev, A, [la.SPMV_ALBUS], [(la.F64, 1e-12)])
+    F = la.iluk(A, 1)
+    P = la.Preconditioner(F, "ilu")
+    r = la.gen.uniform(A.n, 6)
+    zr = orc.prec_apply(_o(A), "ilu", _o(F), r)
+    rt = torch.tensor(r, device=dev)
+    for mode in (la.TRSV_LEVEL, la.TRSV_SYNCFREE):
+        z = torch.zeros_like(rt)
+        P.apply(rt, z, mode=mode)
+        err = rel_err(z.cpu().numpy(), zr)
+        print("ILU(1) apply mode=%d rel err %.3e" % (mode, err), flush=True)
+        assert err < 1e-12
