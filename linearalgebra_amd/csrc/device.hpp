@@ -49,7 +49,7 @@ struct TrsvPlan {
   bool upper = false, unit = true;
   int G = 1;                   // lanes per row
   DevBuf rowptr, col, perm;    // rows in level order; columns keep original ids
-  MultiValues val, diag;       // values / diagonal in level order
+  MultiValues val, diag;       // values / reciprocal diagonal in level order
   std::vector<int> level_ptr;  // host, nlevels+1
   int nchunks = 0;
   DevBuf chunk_start, counter, err_flag;

@@ -18,7 +18,9 @@
 //    ticket, which a running workgroup already holds: the solve always
 //    drains. Waits are bounded; a timeout sets an error flag instead of
 //    hanging.
-// Row formula (triangle_solve.hpp:32-71): x_i = (b_i - sum_j T_ij x_j) / d_i.
+// Row formula (triangle_solve.hpp:32-71): x_i = (b_i - sum_j T_ij x_j) / d_i,
+// computed as a product with 1/d_i stored in the plan (within an ulp of the
+// quotient; keeps a divide off the sync-free dependency chain).
 #include <algorithm>
 #include <cstdlib>
 
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(kB) void k_trsv_level(int k0, int k1, const int *__
   if (sub == 0) {
     const int r = perm[k];
     A v = A(b[r]) - acc;
-    if constexpr (DIAG) v = v / A(ldt(diag, k));
+    if constexpr (DIAG) v = v * A(ldt(diag, k));
     x[r] = V(v);
   }
 }
@@ -134,7 +136,7 @@ __device__ __forceinline__ void sf_row(int k, int sub, const int *__restrict__ r
   acc = group_sum<G>(acc);
   if (sub == 0) {
     A v = bb - acc;
-    if constexpr (DIAG) v = v / dd;
+    if constexpr (DIAG) v = v * dd;
     U o = to_bits<V>(V(v));
     if (o == Sentinel<V>::bits) o = Sentinel<V>::qnan;
     if constexpr (XLOCAL)
@@ -345,7 +347,7 @@ std::unique_ptr<TrsvPlan> make_trsv(int n, int base, const int *ai, const int *a
     std::copy(P->h_aj.begin() + P->h_ai[r], P->h_aj.begin() + P->h_ai[r + 1], cl.begin() + rp[k]);
     std::copy(P->h_av.begin() + P->h_ai[r], P->h_av.begin() + P->h_ai[r + 1], vl.begin() + rp[k]);
     rp[k + 1] = rp[k] + len;
-    if (diag) dg[k] = diag[r];
+    if (diag) dg[k] = 1.0 / diag[r];  // kernels multiply: no divide on the chain
   }
   P->G = pick_G(n ? static_cast<double>(nnz) / n : 1.0);
 
