@@ -126,9 +126,24 @@ def run_pcg(la, torch, A, dev, stream, trsv_mode, variants, cpu=False):
     t_levels = time.time() - t0
     dA = la.DeviceMatrix(A, stream=stream)
     b = torch.tensor(la.gen.uniform(A.n, 2024), device=dev)
+    # the same IC(0) with the numeric phase on the device (bitwise the host's)
+    t0 = time.time()
+    Pg = la.Preconditioner.ic_gpu(A, 0)
+    t_gsetup = time.time() - t0
+    av = torch.tensor(A.av, device=dev)
+    Pg.refactor(av)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Pg.refactor(av)
+    torch.cuda.synchronize()
+    t_gnum = time.perf_counter() - t0
+    same = bool(np.array_equal(Pg.factor().av, R.av))
+    del Pg
     out = {"preconditioner": "IC(0)", "levels": P.levels(), "factor_s": round(t_fact, 3),
-           "analysis_s": round(t_levels, 3), "trsv_mode": ["level", "syncfree"][trsv_mode],
-           "runs": {}}
+           "analysis_s": round(t_levels, 3),
+           "device_factor": {"setup_s": round(t_gsetup, 3), "numeric_s": round(t_gnum, 5),
+                             "bitwise_equal_host": same},
+           "trsv_mode": ["level", "syncfree"][trsv_mode], "runs": {}}
     for name, kw in variants:
         x = torch.zeros(A.n, dtype=torch.float64, device=dev)
         # one untimed run per variant so level graphs and precision copies
@@ -231,6 +246,19 @@ def bench_c4(la, torch, dev, stream, args):
     P = la.Preconditioner(F, "ilu")
     P.set_stream(stream)
     t_an = time.time() - t0
+    # ILU(1) with the numeric phase on the device
+    t0 = time.time()
+    Pg = la.Preconditioner.ilu_gpu(A, 1)
+    t_gsetup = time.time() - t0
+    av = torch.tensor(A.av, device=dev)
+    Pg.refactor(av)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Pg.refactor(av)
+    torch.cuda.synchronize()
+    t_gnum = time.perf_counter() - t0
+    same = bool(np.array_equal(Pg.factor().av, F.av))
+    del Pg, av
     b = torch.tensor(la.gen.uniform(A.n, 4), device=dev)
     mode = la.TRSV_LEVEL if args.trsv == "level" else la.TRSV_SYNCFREE
     runs = {}
@@ -269,7 +297,10 @@ def bench_c4(la, torch, dev, stream, args):
     out = {"config": "c4 aniso27 %d^3" % grid, "n": A.n, "nnz": A.nnz, "spmv": tab,
            "ilu1": {"nnz": int(len(faj)), "levels_L_U": P.levels(), "gen_s": round(t_gen, 2),
                     "symbolic_s": round(t_sym, 2), "numeric_s": round(t_num, 2),
-                    "analysis_upload_s": round(t_an, 2), "apply_ms": ap},
+                    "analysis_upload_s": round(t_an, 2), "apply_ms": ap,
+                    "device_numeric": {"setup_s": round(t_gsetup, 2),
+                                       "numeric_s": round(t_gnum, 4),
+                                       "bitwise_equal_host": same}},
            "trsv_mode": args.trsv, "gmres30": runs}
     f64 = runs["fp64"]["seconds"]
     for k in ("mixed_f32", "mixed_bf16_matrix_f32_factors", "mixed_bf16"):

@@ -228,6 +228,23 @@ int la_prec_create_ilu(int n, int base, const int *fai, const int *faj,
  * 21-29): upper factor R incl. diagonal, M = R^T R. */
 int la_prec_create_ic(int n, int base, const int *rai, const int *raj,
                       const double *rav, la_prec_t *out);
+/* Preconditioners whose numeric factorisation runs on the device (no
+ * counterpart in the reference, whose numeric phase is serial host code:
+ * incomplete_lu.cpp:277-325, precond.hpp:615-693). Creation runs the host
+ * symbolic phase (iluk_symbolic / ICCLevelSymbolic on the given CSR of A;
+ * IC takes the full symmetric A and uses its upper triangle) and builds the
+ * solve plans; la_prec_refactor(P, a_vals) factors on the device from A's
+ * values (device FP64 array in that CSR's order) with the reference's
+ * arithmetic and order -- bitwise the host factor -- and may be called
+ * again with new values. LA_ERR_NUMERIC: missing diagonal (ILU) or negative
+ * pivot (IC). la_prec_factor copies the factor to the host: the pattern
+ * 0-based (fai: n+1, faj/fav: *nnz entries); pass null arrays to get *nnz. */
+int la_prec_create_ilu_gpu(int n, int base, const int *ai, const int *aj, int level,
+                           la_prec_t *out);
+int la_prec_create_ic_gpu(int n, int base, const int *ai, const int *aj, int level, double omega,
+                          la_prec_t *out);
+int la_prec_refactor(la_prec_t P, const double *a_vals);
+int la_prec_factor(la_prec_t P, int *fai, int *faj, double *fav, int64_t *nnz);
 int la_prec_destroy(la_prec_t P);
 int la_prec_add_dtype(la_prec_t P, la_dtype dt);
 int la_prec_set_stream(la_prec_t P, void *hip_stream);

@@ -193,11 +193,47 @@ class Preconditioner:
     """la_prec_t: ILU(k) (combined LU factor) or IC(k) (upper factor R,
     M = R^T R) apply, incomplete_lu.cpp:36-48 / incomplete_cholesky.cpp:21-29."""
 
-    def __init__(self, F, kind):
+    def __init__(self, F, kind, _handle=None):
+        if _handle is not None:
+            self.h, self.n, self.kind = _handle, F, kind
+            return
         h = C.c_void_p()
         fn = {"ilu": lib().la_prec_create_ilu, "ic": lib().la_prec_create_ic}[kind]
         check(fn(F.n, F.base, _p(F.ai), _p(F.aj), _p(F.av), C.byref(h)))
         self.h, self.n, self.kind = h, F.n, kind
+
+    @classmethod
+    def ilu_gpu(cls, A, level):
+        """ILU(level) of A with the numeric phase on the device: call
+        refactor(values) before use (la_prec_create_ilu_gpu)."""
+        h = C.c_void_p()
+        check(lib().la_prec_create_ilu_gpu(A.n, A.base, _p(A.ai), _p(A.aj), level, C.byref(h)))
+        return cls(A.n, "ilu", _handle=h)
+
+    @classmethod
+    def ic_gpu(cls, A, level, omega=0.0):
+        """IC(level) of the symmetric A (its upper triangle), numeric phase on
+        the device (la_prec_create_ic_gpu)."""
+        h = C.c_void_p()
+        check(lib().la_prec_create_ic_gpu(A.n, A.base, _p(A.ai), _p(A.aj), level, float(omega),
+                                          C.byref(h)))
+        return cls(A.n, "ic", _handle=h)
+
+    def refactor(self, a_vals):
+        """Numeric factorisation on the device from A's FP64 values (a
+        device tensor in the CSR order given at creation)."""
+        check(lib().la_prec_refactor(self.h, _dptr(a_vals, F64)))
+        return self
+
+    def factor(self):
+        """The factor (ILU: combined LU; IC: upper R), 0-based host CSR."""
+        nnz = C.c_int64()
+        check(lib().la_prec_factor(self.h, None, None, None, C.byref(nnz)))
+        fai = np.zeros(self.n + 1, np.int32)
+        faj = np.zeros(nnz.value, np.int32)
+        fav = np.zeros(nnz.value)
+        check(lib().la_prec_factor(self.h, _p(fai), _p(faj), _p(fav), C.byref(nnz)))
+        return CSR(self.n, fai, faj, fav)
 
     def __del__(self):
         try:

@@ -92,6 +92,10 @@ _SIGS = {
     "la_trsv_solve_transpose": [_vp, _i, _i, _i, _vp, _vp],
     "la_prec_create_ilu": [_i, _i, _vp, _vp, _vp, C.POINTER(_vp)],
     "la_prec_create_ic": [_i, _i, _vp, _vp, _vp, C.POINTER(_vp)],
+    "la_prec_create_ilu_gpu": [_i, _i, _vp, _vp, _i, C.POINTER(_vp)],
+    "la_prec_create_ic_gpu": [_i, _i, _vp, _vp, _i, _d, C.POINTER(_vp)],
+    "la_prec_refactor": [_vp, _vp],
+    "la_prec_factor": [_vp, _vp, _vp, _vp, C.POINTER(_i64)],
     "la_prec_destroy": [_vp],
     "la_prec_add_dtype": [_vp, _i],
     "la_prec_set_stream": [_vp, _vp],

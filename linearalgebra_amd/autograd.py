@@ -2,9 +2,9 @@
 BINDING over the C ABI, not part of the hot path.
 
 forward: the CSR values `vals` (FP64, device) are written into the device
-matrix, the preconditioner is refactored on the host (IC(0) or ILU(k)
-numeric phase) and uploaded, and the system is solved by PCG (SPD) or
-FGMRES through the C ABI.
+matrix, the preconditioner is refactored on the device (IC(k) or ILU(k)
+numeric phase, la_prec_refactor; the symbolic phase ran once on the host)
+and the system is solved by PCG (SPD) or FGMRES through the C ABI.
 backward: the adjoint system A^T lambda = dL/dx is solved with the
 transposed product and the transposed preconditioner apply
 (la_pcg_solve_transpose / la_fgmres_solve_transpose); then
@@ -35,26 +35,15 @@ class SparseSystem:
         self.opts = dict(rel_tol=rel_tol, max_iters=max_iters, trsv_mode=trsv_mode,
                          precision=precision, inner_tol=inner_tol)
         self.adjoint_rel_tol = rel_tol if adjoint_rel_tol is None else adjoint_rel_tol
-        self.P = None
         self.last = {}
         if kind == "ic":
-            self.U_pattern = la.split_triangle(self.A, upper=True)
-            self.diag_pos = self.U_pattern.ai[:-1]
-            self.icc_ai, self.icc_aj = la.icc_symbolic(self.U_pattern, self.diag_pos, level)
+            self.P = la.Preconditioner.ic_gpu(self.A, level)
         else:
-            self.fai, self.faj, self.fu = la.iluk_symbolic(self.A, level)
+            self.P = la.Preconditioner.ilu_gpu(self.A, level)
 
-    def refactor(self, vals_host):
-        self.A.av = np.ascontiguousarray(vals_host, dtype=np.float64)
-        if self.kind == "ic":
-            U = la.split_triangle(self.A, upper=True)
-            R, ok = la.icc_numeric(U, self.diag_pos, self.icc_ai, self.icc_aj)
-            if not ok:
-                raise la.LaError(5, "IC factorisation hit a negative pivot")
-            self.P = la.Preconditioner(R, "ic")
-        else:
-            F = la.iluk_numeric(self.A, self.fai, self.faj, self.fu)
-            self.P = la.Preconditioner(F, "ilu")
+    def refactor(self, vals):
+        """Numeric factorisation on the device from the FP64 values."""
+        self.P.refactor(vals)
 
     def solve(self, b, transpose=False):
         x = torch.zeros_like(b)
@@ -73,8 +62,9 @@ class SparseSystem:
 class SparseSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vals, b, system):
-        system.dev.set_values(vals.detach().contiguous())
-        system.refactor(vals.detach().cpu().numpy())
+        v = vals.detach().contiguous()
+        system.dev.set_values(v)
+        system.refactor(v)
         x = system.solve(b.detach().contiguous())
         ctx.system = system
         ctx.save_for_backward(x)

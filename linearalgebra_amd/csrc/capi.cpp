@@ -416,6 +416,44 @@ int la_prec_create_ic(int n, int base, const int *rai, const int *raj, const dou
   *out = h.release();
   LA_CATCH
 }
+int la_prec_create_ilu_gpu(int n, int base, const int *ai, const int *aj, int level,
+                           la_prec_t *out) {
+  LA_TRY
+  LA_CHECK_ARG(out && ai && (base == 0 || base == 1) && n >= 0 && level >= 0, "bad argument");
+  LA_CHECK_ARG(ai[0] == base && (ai[n] == base || aj), "bad CSR");
+  auto h = std::make_unique<la_prec_s>();
+  h->p = make_prec_ilu_gpu(n, base, ai, aj, level, nullptr);
+  *out = h.release();
+  LA_CATCH
+}
+int la_prec_create_ic_gpu(int n, int base, const int *ai, const int *aj, int level, double omega,
+                          la_prec_t *out) {
+  LA_TRY
+  LA_CHECK_ARG(out && ai && (base == 0 || base == 1) && n >= 0 && level >= 0, "bad argument");
+  LA_CHECK_ARG(ai[0] == base && (ai[n] == base || aj), "bad CSR");
+  auto h = std::make_unique<la_prec_s>();
+  h->p = make_prec_ic_gpu(n, base, ai, aj, level, omega, nullptr);
+  *out = h.release();
+  LA_CATCH
+}
+int la_prec_refactor(la_prec_t P, const double *a_vals) {
+  LA_TRY
+  LA_CHECK_ARG(P, "null handle");
+  prec_refactor(*P->p, a_vals, P->p->stream);
+  LA_CATCH
+}
+int la_prec_factor(la_prec_t P, int *fai, int *faj, double *fav, int64_t *nnz) {
+  LA_TRY
+  LA_CHECK_ARG(P && nnz, "bad argument");
+  std::vector<int> i, j;
+  std::vector<double> v;
+  prec_factor(*P->p, &i, &j, fav ? &v : nullptr, P->p->stream);
+  *nnz = static_cast<int64_t>(j.size());
+  if (fai) std::copy(i.begin(), i.end(), fai);
+  if (faj) std::copy(j.begin(), j.end(), faj);
+  if (fav) std::copy(v.begin(), v.end(), fav);
+  LA_CATCH
+}
 int la_prec_destroy(la_prec_t P) {
   LA_TRY
   delete P;

@@ -63,6 +63,15 @@ struct TrsvPlan {
   std::vector<int> h_ai, h_aj;
   std::vector<double> h_av, h_diag;
   std::unique_ptr<TrsvPlan> transposed;
+  // level order on the host (position k -> row)
+  std::vector<int> h_perm;
+  // factors computed on the device: where each stored value (original
+  // entry order) and each diagonal entry sits in the factor's value array,
+  // the level-ordered device copies of the same, and the factor array the
+  // values were last gathered from
+  std::vector<int64_t> h_src, h_dsrc;
+  DevBuf val_src, diag_src;
+  const double *fsrc = nullptr;
   // captured per-level launch sequences, keyed by (val, vec, b, x)
   std::map<std::tuple<int, int, const void *, void *>, hipGraphExec_t> graphs;
   ~TrsvPlan();
@@ -75,21 +84,46 @@ void trsv_add_dtype(TrsvPlan &T, la_dtype dt);
 void trsv_solve(TrsvPlan &T, la_trsv_mode mode, la_dtype val, la_dtype vec, const void *b,
                 void *x, hipStream_t s);
 TrsvPlan &trsv_transposed(TrsvPlan &T);
+// records where the plan's values live in a device factor array (src per
+// entry of the triangle as given to make_trsv, dsrc per row; dsrc may be
+// null for a unit diagonal)
+void trsv_set_sources(TrsvPlan &T, const int64_t *src, const int64_t *dsrc, hipStream_t s);
+// gathers the plan's values (and 1/diagonal) from the factor array f, then
+// refreshes the FP32/BF16 copies and any transposed plan
+void trsv_refresh(TrsvPlan &T, const double *f, hipStream_t s);
 // checks (and clears) the sync-free spin-bound error flag; synchronizes s
 bool trsv_check_timeout(TrsvPlan &T, hipStream_t s);
 
 // -------------------------------------------------------------- ILU / IC
+struct NumericPlan;
+struct NumericPlanDeleter {
+  void operator()(NumericPlan *p) const;
+};
 struct Prec {
   bool ic = false;
   int n = 0;
   std::unique_ptr<TrsvPlan> lower, upper;
   DevBuf tmp[2];  // intermediate vector, FP64 and FP32
   hipStream_t stream = nullptr;
+  // set when the factor is computed on the device (make_prec_*_gpu)
+  std::unique_ptr<NumericPlan, NumericPlanDeleter> num;
 };
 std::unique_ptr<Prec> make_prec_ilu(int n, int base, const int *fai, const int *faj,
                                     const double *fav, hipStream_t s);
 std::unique_ptr<Prec> make_prec_ic(int n, int base, const int *rai, const int *raj,
                                    const double *rav, hipStream_t s);
+// device numeric factorisation: symbolic phase on the host at creation,
+// numeric phase (same operation order as the host's, bitwise the same
+// factor) on the device at each prec_refactor
+std::unique_ptr<Prec> make_prec_ilu_gpu(int n, int base, const int *ai, const int *aj, int level,
+                                        hipStream_t s);
+std::unique_ptr<Prec> make_prec_ic_gpu(int n, int base, const int *ai, const int *aj, int level,
+                                       double omega, hipStream_t s);
+// a_vals: device FP64 values of A in the CSR layout given at creation
+void prec_refactor(Prec &P, const double *a_vals, hipStream_t s);
+// factor values (ILU: combined LU; IC: upper R) and pattern, host copies
+void prec_factor(Prec &P, std::vector<int> *fai, std::vector<int> *faj, std::vector<double> *fav,
+                 hipStream_t s);
 void prec_add_dtype(Prec &P, la_dtype dt);
 void prec_apply(Prec &P, la_trsv_mode mode, la_dtype val, la_dtype vec, const void *r, void *z,
                 hipStream_t s);

@@ -337,6 +337,7 @@ std::unique_ptr<TrsvPlan> make_trsv(int n, int base, const int *ai, const int *a
   const int L = topo_sort2(n, 0, P->h_ai.data(), P->h_aj.data(), upper, iperm.data(), prefix.data());
   P->nlevels = L;
   P->level_ptr.assign(prefix.begin(), prefix.begin() + L + 1);
+  P->h_perm.assign(iperm.begin(), iperm.begin() + n);
   // level-ordered copy
   std::vector<int> rp(static_cast<size_t>(n) + 1), cl(static_cast<size_t>(nnz));
   std::vector<double> vl(static_cast<size_t>(nnz)), dg(diag ? n : 0);
@@ -460,10 +461,76 @@ TrsvPlan &trsv_transposed(TrsvPlan &T) {
               tv.data());
     T.transposed = make_trsv(T.n, 0, ti.data(), tj.data(), tv.data(),
                              T.unit ? nullptr : T.h_diag.data(), !T.upper, T.stream);
+    if (!T.h_src.empty() || !T.h_dsrc.empty()) {
+      // the same transpose carries the source positions (exact as doubles)
+      std::vector<double> sv(T.h_src.begin(), T.h_src.end()), st(static_cast<size_t>(T.nnz));
+      std::vector<int> ti2(static_cast<size_t>(T.n) + 1), tj2(static_cast<size_t>(T.nnz));
+      transpose(T.n, T.n, 0, T.h_ai.data(), T.h_aj.data(), sv.data(), ti2.data(), tj2.data(),
+                st.data());
+      std::vector<int64_t> tsrc(st.begin(), st.end());
+      trsv_set_sources(*T.transposed, tsrc.data(), T.h_dsrc.empty() ? nullptr : T.h_dsrc.data(),
+                       T.stream);
+      if (T.fsrc) trsv_refresh(*T.transposed, T.fsrc, T.stream);
+    }
     for (int dt = 1; dt < 3; ++dt)
       if (T.val.has[dt] || T.diag.has[dt]) trsv_add_dtype(*T.transposed, static_cast<la_dtype>(dt));
   }
   return *T.transposed;
+}
+
+namespace {
+__global__ void k_gather(int64_t m, const int64_t *__restrict__ src, const double *__restrict__ f,
+                         double *out) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < m) out[t] = f[src[t]];
+}
+__global__ void k_gather_recip(int64_t m, const int64_t *__restrict__ src,
+                               const double *__restrict__ f, double *out) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < m) out[t] = 1.0 / f[src[t]];
+}
+}  // namespace
+
+void trsv_set_sources(TrsvPlan &T, const int64_t *src, const int64_t *dsrc, hipStream_t s) {
+  const int n = T.n;
+  T.h_src.assign(src, src + T.nnz);
+  if (dsrc) T.h_dsrc.assign(dsrc, dsrc + n);
+  else T.h_dsrc.clear();
+  // level order: entries of row perm[k] in their stored order
+  std::vector<int64_t> ls(static_cast<size_t>(std::max<int64_t>(T.nnz, 1))),
+      ld(static_cast<size_t>(std::max(n, 1)));
+  int64_t p = 0;
+  for (int k = 0; k < n; ++k) {
+    const int r = T.h_perm[k];
+    for (int q = T.h_ai[r]; q < T.h_ai[r + 1]; ++q) ls[p++] = src[q];
+    if (dsrc) ld[k] = dsrc[r];
+  }
+  T.val_src.alloc(sizeof(int64_t) * ls.size());
+  T.val_src.upload(ls.data(), sizeof(int64_t) * ls.size(), s);
+  if (dsrc) {
+    T.diag_src.alloc(sizeof(int64_t) * ld.size());
+    T.diag_src.upload(ld.data(), sizeof(int64_t) * ld.size(), s);
+  }
+  LA_HIP(hipStreamSynchronize(s));
+}
+
+void trsv_refresh(TrsvPlan &T, const double *f, hipStream_t s) {
+  T.fsrc = f;
+  if (T.nnz) {
+    double *v = T.val.v[LA_F64].as<double>();
+    hipLaunchKernelGGL(k_gather, dim3(static_cast<unsigned>((T.nnz + 255) / 256)), dim3(256), 0,
+                       s, T.nnz, T.val_src.as<int64_t>(), f, v);
+    LA_HIP(hipGetLastError());
+    T.val.set_from_device_f64(v, T.nnz, s);
+  }
+  if (!T.unit && !T.h_dsrc.empty() && T.n) {
+    double *d = T.diag.v[LA_F64].as<double>();
+    hipLaunchKernelGGL(k_gather_recip, dim3(static_cast<unsigned>((T.n + 255) / 256)), dim3(256),
+                       0, s, static_cast<int64_t>(T.n), T.diag_src.as<int64_t>(), f, d);
+    LA_HIP(hipGetLastError());
+    T.diag.set_from_device_f64(d, T.n, s);
+  }
+  if (T.transposed) trsv_refresh(*T.transposed, f, s);
 }
 
 bool trsv_check_timeout(TrsvPlan &T, hipStream_t s) {
