@@ -319,6 +319,26 @@ int la_pcg_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
 int la_fgmres_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
                               const double *b, double *x, la_solver_result *res);
 
+/* ====================== readers ======================================= */
+/* Matrix files into a host CSR sorted by (row, column):
+ *  la_read_matrix_market: coordinate real/integer/pattern, general/
+ *    symmetric/skew-symmetric; symmetric files expanded to both triangles
+ *    (read_matrix_market_csr, utils/utils.h:93-132);
+ *  la_read_binary_coo: the reference's int64-count + 16-byte-record COO
+ *    format, indices in `base` (ReadFromBinaryCSR, utils/utils.h:24-91).
+ * Query sizes with la_csr_file_info, copy out with la_csr_file_copy (ai:
+ * rows+1, aj/av: nnz, in the requested base), release with
+ * la_csr_file_free. la_read_binary_vector (ReadFromBinaryVec,
+ * utils/utils.cpp:36-46): out null -> *n = length; else *n = capacity in,
+ * length out. */
+typedef struct la_csr_file_s *la_csr_file_t;
+int la_read_matrix_market(const char *path, int base, la_csr_file_t *out);
+int la_read_binary_coo(const char *path, int base, la_csr_file_t *out);
+int la_csr_file_info(la_csr_file_t F, int *rows, int *cols, int64_t *nnz);
+int la_csr_file_copy(la_csr_file_t F, int *ai, int *aj, double *av);
+int la_csr_file_free(la_csr_file_t F);
+int la_read_binary_vector(const char *path, double *out, int64_t *n);
+
 #ifdef __cplusplus
 }
 #endif

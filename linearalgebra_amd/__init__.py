@@ -9,6 +9,7 @@ vectors are passed as torch tensors (torch only provides device memory and
 streams). Host-side functions take numpy arrays.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -24,6 +25,7 @@ __all__ = [
     "fgmres", "topo_sort2", "split_ldu", "split_triangle", "diagonal",
     "transpose", "permute", "rcm", "iluk_symbolic", "iluk_numeric",
     "icc_symbolic", "icc_numeric", "gen", "F64", "F32", "BF16", "LaError",
+    "read_matrix_market", "read_binary_coo", "read_binary_vector",
 ]
 
 
@@ -306,6 +308,45 @@ def pcg_transpose(A, P, b, x, raise_on_fail=False, **kw):
 
 def fgmres_transpose(A, P, b, x, raise_on_fail=False, **kw):
     return _solve(lib().la_fgmres_solve_transpose, A, P, b, x, raise_on_fail, kw)
+
+
+# ---------------------------------------------------------------- readers
+
+
+def _read_csr(fn, path, base):
+    h = C.c_void_p()
+    check(fn(os.fsencode(path), base, C.byref(h)))
+    try:
+        r, c, nnz = C.c_int(), C.c_int(), C.c_int64()
+        check(lib().la_csr_file_info(h, C.byref(r), C.byref(c), C.byref(nnz)))
+        ai = np.zeros(r.value + 1, np.int32)
+        aj = np.zeros(nnz.value, np.int32)
+        av = np.zeros(nnz.value)
+        check(lib().la_csr_file_copy(h, _p(ai), _p(aj), _p(av)))
+    finally:
+        lib().la_csr_file_free(h)
+    return CSR(r.value, ai, aj, av, ncols=c.value)
+
+
+def read_matrix_market(path, base=0):
+    """Matrix Market coordinate file -> CSR sorted by (row, col), symmetric
+    files expanded (read_matrix_market_csr, utils/utils.h:93-132)."""
+    return _read_csr(lib().la_read_matrix_market, path, base)
+
+
+def read_binary_coo(path, base=1):
+    """The reference's binary COO file -> CSR in `base`
+    (ReadFromBinaryCSR, utils/utils.h:24-91)."""
+    return _read_csr(lib().la_read_binary_coo, path, base)
+
+
+def read_binary_vector(path):
+    """ReadFromBinaryVec (utils/utils.cpp:36-46)."""
+    n = C.c_int64()
+    check(lib().la_read_binary_vector(os.fsencode(path), None, C.byref(n)))
+    out = np.zeros(n.value)
+    check(lib().la_read_binary_vector(os.fsencode(path), _p(out), C.byref(n)))
+    return out
 
 
 # ---------------------------------------------------------------- host

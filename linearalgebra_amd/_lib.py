@@ -111,6 +111,12 @@ _SIGS = {
                                C.POINTER(SolverResult)],
     "la_fgmres_solve_transpose": [_vp, _vp, C.POINTER(SolverOpts), _vp, _vp,
                                   C.POINTER(SolverResult)],
+    "la_read_matrix_market": [C.c_char_p, _i, C.POINTER(_vp)],
+    "la_read_binary_coo": [C.c_char_p, _i, C.POINTER(_vp)],
+    "la_csr_file_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i64)],
+    "la_csr_file_copy": [_vp, _vp, _vp, _vp],
+    "la_csr_file_free": [_vp],
+    "la_read_binary_vector": [C.c_char_p, _vp, C.POINTER(_i64)],
 }
 _RESTYPES = {"la_last_error": C.c_char_p, "la_solver_opts_default": None}
 

@@ -17,7 +17,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblaamd.so")
 
 SOURCES = ["host_algos.cpp", "gen.cpp", "spmv.hip", "sptrsv.hip", "numeric.hip", "blas1.hip",
-           "solvers.cpp", "capi.cpp"]
+           "solvers.cpp", "io.cpp", "capi.cpp"]
 
 
 def hipcc():

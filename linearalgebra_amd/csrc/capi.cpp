@@ -44,6 +44,18 @@ struct la_trsv_s {
 struct la_prec_s {
   std::unique_ptr<Prec> p;
 };
+struct la_csr_file_s {
+  int rows = 0, cols = 0;
+  std::vector<int> ai, aj;
+  std::vector<double> av;
+};
+namespace laamd {
+void read_matrix_market(const char *path, int base, int *rows, int *cols, std::vector<int> &ai,
+                        std::vector<int> &aj, std::vector<double> &av);
+void read_binary_coo_csr(const char *path, int base, int *rows, int *cols, std::vector<int> &ai,
+                         std::vector<int> &aj, std::vector<double> &av);
+void read_binary_vector(const char *path, std::vector<double> &v);
+}  // namespace laamd
 
 #define LA_TRY try {
 #define LA_CATCH                                                   \
@@ -600,6 +612,57 @@ int la_fgmres_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opt
     set_last_error("FGMRES (transposed) did not reach the tolerance within the iteration limit");
     return st;
   }
+  LA_CATCH
+}
+
+// ------------------------------------------------------------ readers
+int la_read_matrix_market(const char *path, int base, la_csr_file_t *out) {
+  LA_TRY
+  LA_CHECK_ARG(out, "null output");
+  auto h = std::make_unique<la_csr_file_s>();
+  read_matrix_market(path, base, &h->rows, &h->cols, h->ai, h->aj, h->av);
+  *out = h.release();
+  LA_CATCH
+}
+int la_read_binary_coo(const char *path, int base, la_csr_file_t *out) {
+  LA_TRY
+  LA_CHECK_ARG(out, "null output");
+  auto h = std::make_unique<la_csr_file_s>();
+  read_binary_coo_csr(path, base, &h->rows, &h->cols, h->ai, h->aj, h->av);
+  *out = h.release();
+  LA_CATCH
+}
+int la_csr_file_info(la_csr_file_t F, int *rows, int *cols, int64_t *nnz) {
+  LA_TRY
+  LA_CHECK_ARG(F, "null handle");
+  if (rows) *rows = F->rows;
+  if (cols) *cols = F->cols;
+  if (nnz) *nnz = static_cast<int64_t>(F->aj.size());
+  LA_CATCH
+}
+int la_csr_file_copy(la_csr_file_t F, int *ai, int *aj, double *av) {
+  LA_TRY
+  LA_CHECK_ARG(F, "null handle");
+  if (ai) std::copy(F->ai.begin(), F->ai.end(), ai);
+  if (aj) std::copy(F->aj.begin(), F->aj.end(), aj);
+  if (av) std::copy(F->av.begin(), F->av.end(), av);
+  LA_CATCH
+}
+int la_csr_file_free(la_csr_file_t F) {
+  LA_TRY
+  delete F;
+  LA_CATCH
+}
+int la_read_binary_vector(const char *path, double *out, int64_t *n) {
+  LA_TRY
+  LA_CHECK_ARG(n, "null size");
+  std::vector<double> v;
+  read_binary_vector(path, v);
+  if (out) {
+    LA_CHECK_ARG(*n >= static_cast<int64_t>(v.size()), "output too small");
+    std::copy(v.begin(), v.end(), out);
+  }
+  *n = static_cast<int64_t>(v.size());
   LA_CATCH
 }
 

@@ -27,12 +27,21 @@ def _cases():
           ("empty_rows", lambda: from_dense(_empty_rows())),
           ("one_long_row", lambda: from_dense(_one_long_row())),
           ("all_empty", lambda: orc.CSR(7, np.zeros(8), [], [])),
-          ("one_by_one", lambda: from_dense([[3.0]]))]
+          ("one_by_one", lambda: from_dense([[3.0]])),
+          ("K_bin_base1", lambda: _bin("K.bin")),
+          ("G2_bin_rect", lambda: _bin("G2.bin"))]
     return cs
 
 
 def _o(A):
     return orc.CSR(A.n, A.ai, A.aj, A.av, ncols=A.ncols)
+
+
+def _bin(name):
+    """the reference's binary COO fixtures, read 1-based as its tests do"""
+    import os
+    from helpers import GOLDEN
+    return _o(la.read_binary_coo(os.path.join(GOLDEN, name), base=1))
 
 
 def _empty_rows():
