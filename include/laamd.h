@@ -319,6 +319,17 @@ int la_pcg_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
 int la_fgmres_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
                               const double *b, double *x, la_solver_result *res);
 
+/* ====================== symmetric scaling ============================= */
+/* rowwiseSqrtNorm (mkl_wrapper/mkl_sparse_mat.cpp:517-533): d_i =
+ * 1/sqrt(sqrt(sum_j a_ij^2)) (1 when the sum is not above DBL_MIN);
+ * DtAD (:535-544): a_ij *= d_i * d_j, in place. Host arrays: la_*; a device
+ * matrix and a device d: la_mat_* (bitwise the host results; la_mat_dtad
+ * rescales the matrix's FP32/BF16 copies too). */
+int la_rowwise_sqrt_norm(int n, int base, const int *ai, const double *av, double *d);
+int la_dtad(int n, int base, const int *ai, const int *aj, double *av, const double *d);
+int la_mat_rowwise_sqrt_norm(la_mat_t A, double *d);
+int la_mat_dtad(la_mat_t A, const double *d);
+
 /* ====================== readers ======================================= */
 /* Matrix files into a host CSR sorted by (row, column):
  *  la_read_matrix_market: coordinate real/integer/pattern, general/

@@ -25,7 +25,7 @@ __all__ = [
     "fgmres", "topo_sort2", "split_ldu", "split_triangle", "diagonal",
     "transpose", "permute", "rcm", "iluk_symbolic", "iluk_numeric",
     "icc_symbolic", "icc_numeric", "gen", "F64", "F32", "BF16", "LaError",
-    "read_matrix_market", "read_binary_coo", "read_binary_vector",
+    "read_matrix_market", "read_binary_coo", "read_binary_vector", "rowwise_sqrt_norm", "dtad",
 ]
 
 
@@ -149,6 +149,16 @@ class DeviceMatrix:
     def value_grad(self, lam, x, g):
         check(lib().la_csr_value_grad(self.h, _dptr(lam, F64), _dptr(x, F64), _dptr(g, F64)))
         return g
+
+    def rowwise_sqrt_norm(self, d):
+        """rowwiseSqrtNorm of the device values into the device tensor d."""
+        check(lib().la_mat_rowwise_sqrt_norm(self.h, _dptr(d, F64)))
+        return d
+
+    def dtad(self, d):
+        """DtAD in place on the device values (d: device tensor)."""
+        check(lib().la_mat_dtad(self.h, _dptr(d, F64)))
+        return self
 
 
 class TriangularSolve:
@@ -308,6 +318,23 @@ def pcg_transpose(A, P, b, x, raise_on_fail=False, **kw):
 
 def fgmres_transpose(A, P, b, x, raise_on_fail=False, **kw):
     return _solve(lib().la_fgmres_solve_transpose, A, P, b, x, raise_on_fail, kw)
+
+
+# ---------------------------------------------------------------- scaling
+
+
+def rowwise_sqrt_norm(A):
+    """mkl_sparse_mat::rowwiseSqrtNorm (mkl_sparse_mat.cpp:517-533)."""
+    d = np.zeros(A.n)
+    check(lib().la_rowwise_sqrt_norm(A.n, A.base, _p(A.ai), _p(A.av), _p(d)))
+    return d
+
+
+def dtad(A, d):
+    """mkl_sparse_mat::DtAD (mkl_sparse_mat.cpp:535-544) -> a new CSR."""
+    av = np.array(A.av, dtype=np.float64)
+    check(lib().la_dtad(A.n, A.base, _p(A.ai), _p(A.aj), _p(av), _p(_f64(d))))
+    return CSR(A.n, A.ai, A.aj, av, ncols=A.ncols)
 
 
 # ---------------------------------------------------------------- readers

@@ -111,6 +111,10 @@ _SIGS = {
                                C.POINTER(SolverResult)],
     "la_fgmres_solve_transpose": [_vp, _vp, C.POINTER(SolverOpts), _vp, _vp,
                                   C.POINTER(SolverResult)],
+    "la_rowwise_sqrt_norm": [_i, _i, _vp, _vp, _vp],
+    "la_dtad": [_i, _i, _vp, _vp, _vp, _vp],
+    "la_mat_rowwise_sqrt_norm": [_vp, _vp],
+    "la_mat_dtad": [_vp, _vp],
     "la_read_matrix_market": [C.c_char_p, _i, C.POINTER(_vp)],
     "la_read_binary_coo": [C.c_char_p, _i, C.POINTER(_vp)],
     "la_csr_file_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i64)],

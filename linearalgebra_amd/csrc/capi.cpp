@@ -615,6 +615,38 @@ int la_fgmres_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opt
   LA_CATCH
 }
 
+// ------------------------------------------------------------ scaling
+int la_rowwise_sqrt_norm(int n, int base, const int *ai, const double *av, double *d) {
+  LA_TRY
+  LA_CHECK_ARG(n >= 0 && ai && d && (base == 0 || base == 1), "bad argument");
+  LA_CHECK_ARG(ai[n] == ai[0] || av, "null values");
+  rowwise_sqrt_norm(n, base, ai, av, d);
+  LA_CATCH
+}
+int la_dtad(int n, int base, const int *ai, const int *aj, double *av, const double *d) {
+  LA_TRY
+  LA_CHECK_ARG(n >= 0 && ai && d && (base == 0 || base == 1), "bad argument");
+  LA_CHECK_ARG(ai[n] == ai[0] || (aj && av), "null arrays");
+  for (int64_t q = 0; q < static_cast<int64_t>(ai[n]) - ai[0]; ++q)
+    LA_CHECK_ARG(aj[q] - base >= 0 && aj[q] - base < n, "column out of range");
+  dtad(n, base, ai, aj, av, d);
+  LA_CATCH
+}
+int la_mat_rowwise_sqrt_norm(la_mat_t A, double *d) {
+  LA_TRY
+  LA_CHECK_ARG(A && (d || A->m->rows == 0), "bad argument");
+  mat_rowwise_sqrt_norm(*A->m, d, A->m->stream);
+  LA_HIP(hipStreamSynchronize(A->m->stream));
+  LA_CATCH
+}
+int la_mat_dtad(la_mat_t A, const double *d) {
+  LA_TRY
+  LA_CHECK_ARG(A && (d || A->m->rows == 0), "bad argument");
+  mat_dtad(*A->m, d, A->m->stream);
+  LA_HIP(hipStreamSynchronize(A->m->stream));
+  LA_CATCH
+}
+
 // ------------------------------------------------------------ readers
 int la_read_matrix_market(const char *path, int base, la_csr_file_t *out) {
   LA_TRY

@@ -32,6 +32,9 @@ struct Mat {
 std::unique_ptr<Mat> make_mat(int rows, int cols, int base, const int *ai, const int *aj,
                               const double *av, hipStream_t s);
 void spmv_prepare_segments(Mat &A);
+// rowwiseSqrtNorm / DtAD on the device matrix (d: device, rows entries)
+void mat_rowwise_sqrt_norm(Mat &A, double *d, hipStream_t s);
+void mat_dtad(Mat &A, const double *d, hipStream_t s);
 // y = alpha*A*x + beta*z (z may equal y; beta == 0 never reads z)
 void spmv(Mat &A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, double alpha,
           const void *x, double beta, const void *z, void *y, hipStream_t s);

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <limits>
 #include <numeric>
 
 namespace laamd {
@@ -78,6 +79,23 @@ void split_triangle(int n, int base, const int *ai, const int *aj,
     Tai[i + 1] = static_cast<int>(p + base);
   }
   if (tnnz) *tnnz = p;
+}
+
+// mkl_wrapper/mkl_sparse_mat.cpp:517-533 (rowwiseSqrtNorm): d_i =
+// 1 / sqrt(sqrt(sum_j a_ij^2)), or 1 when that sum is not above DBL_MIN;
+// the sum runs left to right over the row.
+void rowwise_sqrt_norm(int n, int base, const int *ai, const double *av, double *d) {
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int64_t j = ai[i] - base; j < ai[i + 1] - base; ++j) s += av[j] * av[j];
+    d[i] = s > std::numeric_limits<double>::min() ? 1.0 / std::sqrt(std::sqrt(s)) : 1.0;
+  }
+}
+
+// mkl_wrapper/mkl_sparse_mat.cpp:535-544 (DtAD): a_ij *= d_i * d_j.
+void dtad(int n, int base, const int *ai, const int *aj, double *av, const double *d) {
+  for (int i = 0; i < n; ++i)
+    for (int64_t j = ai[i] - base; j < ai[i + 1] - base; ++j) av[j] *= d[i] * d[aj[j] - base];
 }
 
 // sparse_mat_op/matrix_utils.hpp:677-704 (Diagonal).

@@ -13,6 +13,8 @@ void split_ldu(int n, int base, const int *ai, const int *aj, const double *av,
 void split_triangle(int n, int base, const int *ai, const int *aj,
                     const double *av, bool upper, int *Tai, int *Taj,
                     double *Tav, int64_t *tnnz);
+void rowwise_sqrt_norm(int n, int base, const int *ai, const double *av, double *d);
+void dtad(int n, int base, const int *ai, const int *aj, double *av, const double *d);
 bool diagonal(int n, int base, const int *ai, const int *aj, const double *av,
               int *diagpos, double *diag);
 void transpose(int rows, int cols, int base, const int *ai, const int *aj,

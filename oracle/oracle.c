@@ -9,6 +9,7 @@
  */
 #include "oracle.h"
 
+#include <float.h>
 #include <limits.h>
 #include <math.h>
 #include <omp.h>
@@ -686,6 +687,24 @@ static double dot(int n, const double *a, const double *b) {
   double s = 0;
   for (int i = 0; i < n; i++) s += a[i] * b[i];
   return s;
+}
+
+/* mkl_wrapper/mkl_sparse_mat.cpp:517-533 */
+void orc_rowwise_sqrt_norm(int n, int base, const int *ai, const double *av,
+                           double *d) {
+  for (int i = 0; i < n; i++) {
+    double s = 0;
+    for (int j = ai[i] - base; j != ai[i + 1] - base; j++) s += av[j] * av[j];
+    d[i] = s > DBL_MIN ? 1. / sqrt(sqrt(s)) : 1.;
+  }
+}
+
+/* mkl_wrapper/mkl_sparse_mat.cpp:535-544 */
+void orc_dtad(int n, int base, const int *ai, const int *aj, double *av,
+              const double *d) {
+  for (int i = 0; i < n; i++)
+    for (int j = ai[i] - base; j != ai[i + 1] - base; j++)
+      av[j] *= d[i] * d[aj[j] - base];
 }
 
 /* ILU apply: trsv(L unit) then trsv(U non-unit) on the combined factor

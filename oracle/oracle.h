@@ -154,6 +154,16 @@ int orc_fgmres(int n, int base, const int *ai, const int *aj,
 void orc_prec_apply(int n, int base, const orc_prec *P, const double *r,
                     double *z);
 
+/* ---- symmetric scaling (mkl_wrapper/mkl_sparse_mat.cpp:517-544) ---- */
+/* rowwiseSqrtNorm: d_i = 1/sqrt(sqrt(sum_j a_ij^2)) (1 if the sum is not
+ * above DBL_MIN), summed left to right. Pinned by a Python loop in the same
+ * order and by d_i^4 * sum_j a_ij^2 == 1 within rounding. */
+void orc_rowwise_sqrt_norm(int n, int base, const int *ai, const double *av,
+                           double *d);
+/* DtAD: a_ij *= d_i * d_j in place. Pinned by the dense D A D product. */
+void orc_dtad(int n, int base, const int *ai, const int *aj, double *av,
+              const double *d);
+
 /* ---- RCM (reordering/Reordering.cpp:37-136 and what it calls) ---- */
 /* iperm[new] = old + base, perm = inverse. The reference's BFS width member
  * is uninitialised (reordering/BFS.h:47); here it starts at 0. Exact

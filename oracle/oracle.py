@@ -78,6 +78,8 @@ def _declare(L):
                              d, i, C.POINTER(i), C.POINTER(d)]
     L.orc_fgmres.restype = i
     L.orc_prec_apply.argtypes = [i, i, C.POINTER(_Prec), _D, _D]
+    L.orc_rowwise_sqrt_norm.argtypes = [i, i, _I, _D, _D]
+    L.orc_dtad.argtypes = [i, i, _I, _I, _D, _D]
     L.orc_rcm.argtypes = [i, i, _I, _I, _I, _I]
     L.orc_bfs_levels.argtypes = [i, i, _I, _I, i, _I]
     L.orc_bfs_levels.restype = i
@@ -305,6 +307,18 @@ def fgmres(A, b, x0=None, kind="none", F=None, restart=30, rel_tol=1e-8,
                           x, restart, rel_tol, abs_tol, maxiter, C.byref(it),
                           C.byref(rr))
     return x, bool(ok), it.value, rr.value
+
+
+def rowwise_sqrt_norm(A):
+    d = np.zeros(A.n)
+    lib().orc_rowwise_sqrt_norm(A.n, A.base, A.ai, A.av, d)
+    return d
+
+
+def dtad(A, d):
+    av = A.av.copy()
+    lib().orc_dtad(A.n, A.base, A.ai, A.aj, av, _f64(d))
+    return CSR(A.n, A.ai, A.aj, av, ncols=A.ncols)
 
 
 def rcm(A):
