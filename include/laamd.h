@@ -314,6 +314,13 @@ int la_fgmres_solve(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
 /* A^T x = b with the same loops, the transposed product
  * (mkl_sparse_mat::transpose_mult_vec, mkl_sparse_mat.cpp:331-336) and the
  * transposed preconditioner apply M^-T; the adjoint solve of x = A^-1 b. */
+/* FGMRES with a right operator R (mkl_fgmres_solver(A, P, R),
+ * mkl_wrapper/mkl_solver.cpp:319-324): the Krylov operator is A R^-1 (R
+ * applied as a preconditioner, then A) and P the flexible preconditioner;
+ * x returns u with A R^-1 u = b, as the reference does (the solution of
+ * A x = b is R^-1 u). FP64 precision only. */
+int la_fgmres_solve_right(la_mat_t A, la_prec_t P, la_prec_t R, const la_solver_opts *opts,
+                          const double *b, double *x, la_solver_result *res);
 int la_pcg_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
                            const double *b, double *x, la_solver_result *res);
 int la_fgmres_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts,

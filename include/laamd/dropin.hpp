@@ -442,6 +442,18 @@ class mkl_sparse_mat {
     dx.download(x, sizeof(double) * _ncol);
     return true;
   }
+  // mkl_sparse_mat::rowwiseSqrtNorm (mkl_sparse_mat.cpp:517-533)
+  std::vector<double> rowwiseSqrtNorm() const {
+    std::vector<double> d(_nrow);
+    detail::check(la_rowwise_sqrt_norm(_nrow, _base, _ai.get(), _av.get(), d.data()));
+    return d;
+  }
+  // mkl_sparse_mat::DtAD (mkl_sparse_mat.cpp:535-544): a_ij *= d_i d_j, in
+  // place; the device copy is dropped and re-uploaded on next use
+  void DtAD(const std::vector<double> &diag) {
+    detail::check(la_dtad(_nrow, _base, _ai.get(), _aj.get(), _av.get(), diag.data()));
+    _dev.reset();
+  }
   // preconditioners override this (z = M^-1 r)
   virtual bool solve(double const *const, double *const) { return false; }
   virtual bool solve_device(double const *const, double *const) { return false; }
@@ -706,15 +718,15 @@ class mkl_pcg_solver : public mkl_iterative_solver {
   std::shared_ptr<mkl_sparse_mat> _P;
 };
 
-// mkl_fgmres_solver (mkl_solver.h:72-87, mkl_solver.cpp:253-386). The
-// right operator _R of the reference is not supported (must be null).
+// mkl_fgmres_solver (mkl_solver.h:72-87, mkl_solver.cpp:253-386), with the
+// reference's optional right operator _R.
 class mkl_fgmres_solver : public mkl_iterative_solver {
  public:
+  // R: right operator (mkl_solver.cpp:319-324): the Krylov operator is
+  // A R^-1 and x returns u with A R^-1 u = b, as in the reference
   mkl_fgmres_solver(mkl_sparse_mat *A, std::shared_ptr<mkl_sparse_mat> P = nullptr,
                     std::shared_ptr<mkl_sparse_mat> R = nullptr)
-      : _A(A), _P(P) {
-    if (R) throw error(LA_ERR_ARG, "laamd: FGMRES right operator is not supported");
-  }
+      : _A(A), _P(P), _R(R) {}
   void set_restart_steps(int n) { _num_restart = n; }
   bool solve(double const *const b, double *const x) override {
     if (!_A) return false;
@@ -732,10 +744,13 @@ class mkl_fgmres_solver : public mkl_iterative_solver {
     la_prec_t P = detail_solver::prec_of(_P);
     la_solver_opts o = opts(P, _P.get());
     o.restart = _num_restart > 0 ? _num_restart : std::min(_maxiter, 63);
+    if (_R)
+      return la_fgmres_solve_right(detail_solver::device_matrix(_A), P,
+                                   detail_solver::prec_of(_R), &o, b, x, r);
     return la_fgmres_solve(detail_solver::device_matrix(_A), P, &o, b, x, r);
   }
   mkl_sparse_mat *_A;
-  std::shared_ptr<mkl_sparse_mat> _P;
+  std::shared_ptr<mkl_sparse_mat> _P, _R;
   int _num_restart = 0;
 };
 

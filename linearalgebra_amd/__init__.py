@@ -316,6 +316,19 @@ def pcg_transpose(A, P, b, x, raise_on_fail=False, **kw):
     return _solve(lib().la_pcg_solve_transpose, A, P, b, x, raise_on_fail, kw)
 
 
+def fgmres_right(A, P, R, b, x, raise_on_fail=False, **kw):
+    """mkl_fgmres_solver(A, P, R)::solve (mkl_solver.cpp:253-386 with the
+    right operator of :319-324): x <- u with A R^-1 u = b."""
+    o = solver_opts(**kw)
+    r = SolverResult()
+    st = lib().la_fgmres_solve_right(A.h, None if P is None else P.h, R.h, C.byref(o),
+                                     _dptr(b, F64), _dptr(x, F64), C.byref(r))
+    if st != 0 and (raise_on_fail or st != _lib.LA_ERR_NOT_CONVERGED):
+        check(st)
+    return {"converged": bool(r.converged), "iters": r.iters, "outer_iters": r.outer_iters,
+            "rel_residual": r.rel_residual, "seconds": r.seconds, "status": st}
+
+
 def fgmres_transpose(A, P, b, x, raise_on_fail=False, **kw):
     return _solve(lib().la_fgmres_solve_transpose, A, P, b, x, raise_on_fail, kw)
 

@@ -111,6 +111,8 @@ _SIGS = {
                                C.POINTER(SolverResult)],
     "la_fgmres_solve_transpose": [_vp, _vp, C.POINTER(SolverOpts), _vp, _vp,
                                   C.POINTER(SolverResult)],
+    "la_fgmres_solve_right": [_vp, _vp, _vp, C.POINTER(SolverOpts), _vp, _vp,
+                              C.POINTER(SolverResult)],
     "la_rowwise_sqrt_norm": [_i, _i, _vp, _vp, _vp],
     "la_dtad": [_i, _i, _vp, _vp, _vp, _vp],
     "la_mat_rowwise_sqrt_norm": [_vp, _vp],
@@ -135,6 +137,14 @@ def load(path=LIB_PATH):
     if not os.path.exists(path):
         raise ImportError("liblaamd.so not built: run `python -m linearalgebra_amd.build` "
                           "(or __graft_entry__.build()) first; no fallback exists")
+    # torch carries its own HIP runtime, and the two runtimes in one process
+    # share one HSA runtime: torch's must come up first, or this library's
+    # later device calls find no device. Bring it up before loading.
+    try:
+        import torch
+        torch.cuda.is_available()
+    except ImportError:
+        pass
     L = C.CDLL(path)
     for name, args in _SIGS.items():
         f = getattr(L, name)

@@ -14,6 +14,8 @@ int fgmres_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, doub
                  la_solver_result *res);
 int pcg_solve_transpose(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
                         la_solver_result *res);
+int fgmres_solve_right(Mat &A, Prec *P, Prec *R, const la_solver_opts &o, const double *b,
+                       double *x, la_solver_result *res);
 int fgmres_solve_transpose(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
                            la_solver_result *res);
 void gen_uniform(int64_t n, uint64_t seed, double lo, double hi, double *out);
@@ -602,6 +604,17 @@ int la_pcg_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts, 
   LA_CATCH
 }
 
+int la_fgmres_solve_right(la_mat_t A, la_prec_t P, la_prec_t R, const la_solver_opts *opts,
+                          const double *b, double *x, la_solver_result *res) {
+  LA_TRY
+  LA_CHECK_ARG(A && R && opts && b && x, "bad argument");
+  const int st = fgmres_solve_right(*A->m, P ? P->p.get() : nullptr, R->p.get(), *opts, b, x, res);
+  if (st == LA_ERR_NOT_CONVERGED) {
+    set_last_error("FGMRES (right operator) did not reach the tolerance within the iteration limit");
+    return st;
+  }
+  LA_CATCH
+}
 int la_fgmres_solve_transpose(la_mat_t A, la_prec_t P, const la_solver_opts *opts,
                               const double *b, double *x, la_solver_result *res) {
   LA_TRY

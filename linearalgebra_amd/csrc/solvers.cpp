@@ -25,7 +25,7 @@ namespace {
 struct Work {
   int64_t n = 0;
   int m = 0;
-  DevBuf v64[6], v32[6];
+  DevBuf v64[7], v32[7];
   DevBuf V, Z;  // GMRES bases
   la_dtype basis_dt = LA_F64;
   Scratch sc;
@@ -37,7 +37,7 @@ Work &work_for(int64_t n) {
   if (!tl_work || tl_work->n != n) {
     tl_work = std::make_unique<Work>();
     tl_work->n = n;
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < 7; ++i) {
       tl_work->v64[i].alloc(sizeof(double) * std::max<int64_t>(n, 1));
       tl_work->v32[i].alloc(sizeof(float) * std::max<int64_t>(n, 1));
     }
@@ -68,6 +68,9 @@ la_dtype inner_prec_dtype(int precision) {
 
 // z = M^-1 r, or M^-T r when the solve is with A^T
 thread_local bool tl_transpose = false;
+// right operator of FGMRES (mkl_fgmres_solver's _R, mkl_solver.cpp:319-324):
+// the Krylov operator is A R^-1
+thread_local Prec *tl_right = nullptr;
 void apply_M(Prec *P, const la_solver_opts &o, la_dtype val, la_dtype vec, const void *r,
              void *z, int64_t n, hipStream_t s) {
   const la_trsv_mode m = static_cast<la_trsv_mode>(o.trsv_mode);
@@ -75,6 +78,20 @@ void apply_M(Prec *P, const la_solver_opts &o, la_dtype val, la_dtype vec, const
   if (!P) copy_vec(n, vec, r, vec, z, 1.0, s);
   else if (tl_transpose) prec_apply_transpose(*P, m, val, vec, r, z, s);
   else prec_apply(*P, m, val, vec, r, z, s);
+}
+
+// y = alpha * Op x + beta * z with Op = A, or A R^-1 under a right operator
+// (R^-1 x goes through the spare work vector 6)
+void op_apply(Mat &A, const la_solver_opts &o, la_spmv_alg alg, la_dtype val, la_dtype vec,
+              la_dtype acc, double alpha, const void *x, double beta, const void *z, void *y,
+              Work &W, hipStream_t s) {
+  if (!tl_right) {
+    spmv(A, alg, val, vec, acc, alpha, x, beta, z, y, s);
+    return;
+  }
+  void *t = (vec == LA_F64 ? W.v64 : W.v32)[6].p;
+  prec_apply(*tl_right, static_cast<la_trsv_mode>(o.trsv_mode), val, vec, x, t, s);
+  spmv(A, alg, val, vec, acc, alpha, t, beta, z, y, s);
 }
 
 // One PCG run on (val, vec) precision. x holds x0; vectors b/x in `vec`.
@@ -162,7 +179,7 @@ CgResult fgmres_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_
   int total = 0;
   bool done = false;
   while (!done) {
-    spmv(A, alg, val, vec, acc, -1.0, x, 1.0, b, w, s);
+    op_apply(A, o, alg, val, vec, acc, -1.0, x, 1.0, b, w, W, s);
     const double beta = norm2(n, vec, w, W.sc, s);
     if (beta == 0.0 || beta / nb < rel_tol || beta < abs_tol) {
       R.converged = true;
@@ -175,7 +192,7 @@ CgResult fgmres_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_
     int k = 0;
     for (int j = 0; j < m; ++j) {
       apply_M(P, o, val, vec, Vj(j), Zj(j), n, s);
-      spmv(A, alg, val, vec, acc, 1.0, Zj(j), 0.0, w, w, s);
+      op_apply(A, o, alg, val, vec, acc, 1.0, Zj(j), 0.0, w, w, W, s);
       const int64_t ld = n;
       multidot_dev(n, j + 1, vec, V, ld, w, part, h1, s);
       multiaxpy_dev(n, j + 1, vec, V, ld, h1, w, s);
@@ -262,7 +279,8 @@ int run_solver(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double
     int total = 0, outer = 0;
     bool conv = false;
     for (outer = 0; outer < std::max(1, o.max_outer); ++outer) {
-      spmv(A, static_cast<la_spmv_alg>(o.spmv_alg), LA_F64, LA_F64, LA_F64, -1.0, x, 1.0, b, r64, s);
+      op_apply(A, o, static_cast<la_spmv_alg>(o.spmv_alg), LA_F64, LA_F64, LA_F64, -1.0, x, 1.0,
+               b, r64, W, s);
       const double rn = norm2(n, LA_F64, r64, W.sc, s);
       if (o.print_level == 1) std::fprintf(stderr, "outer %d: %lg/%lg=%lg\n", outer, rn, nb, rn / nb);
       if (rn / nb < o.rel_tol || rn < o.abs_tol) { conv = true; break; }
@@ -287,14 +305,14 @@ int run_solver(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double
   out.seconds = std::chrono::duration<double>(clk::now() - t0).count();
   // final true residual in FP64
   double *t = W.v64[4].as<double>();
-  spmv(A, LA_SPMV_AUTO, LA_F64, LA_F64, LA_F64, -1.0, x, 1.0, b, t, s);
+  op_apply(A, o, LA_SPMV_AUTO, LA_F64, LA_F64, LA_F64, -1.0, x, 1.0, b, t, W, s);
   const double tr = norm2(n, LA_F64, t, W.sc, s);
   out.rel_residual = nb > 0 ? tr / nb : 0.0;
   if (o.precision == LA_PREC_FP64 && out.converged && o.stop_rule == LA_STOP_RECURRENCE &&
       !(out.rel_residual < o.rel_tol || tr < o.abs_tol * 1.0000001))
     out.converged = 0;  // recurrence said yes, the true residual says no
-  if (P) {
-    if (trsv_check_timeout(*P->lower, s) || trsv_check_timeout(*P->upper, s))
+  for (Prec *Q : {P, tl_right}) {
+    if (Q && (trsv_check_timeout(*Q->lower, s) || trsv_check_timeout(*Q->upper, s)))
       throw la_error(LA_ERR_TIMEOUT, "sync-free triangular solve exceeded its spin bound");
   }
   if (res) *res = out;
@@ -334,6 +352,20 @@ int fgmres_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, doub
                         double rt, double at, int mi, Work &W) {
                       return fgmres_core(A, P, o, val, vec, bb, xx, nb, rt, at, mi, m, W, s);
                     });
+}
+
+// FGMRES on A R^-1 u = b (mkl_fgmres_solver with a right operator R,
+// mkl_solver.cpp:319-324): returns u, as the reference does; the solution
+// of A x = b is R^-1 u. FP64 only.
+int fgmres_solve_right(Mat &A, Prec *P, Prec *R, const la_solver_opts &o, const double *b,
+                       double *x, la_solver_result *res) {
+  LA_CHECK_ARG(R && R->n == A.rows, "right operator size mismatch");
+  LA_CHECK_ARG(o.precision == LA_PREC_FP64, "a right operator runs in FP64 only");
+  struct RightScope {
+    explicit RightScope(Prec *r) { tl_right = r; }
+    ~RightScope() { tl_right = nullptr; }
+  } scope(R);
+  return fgmres_solve(A, P, o, b, x, res);
 }
 
 }  // namespace laamd

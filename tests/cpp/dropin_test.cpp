@@ -127,6 +127,39 @@ int main() {
   ok = gm && gm->solve(rhs.data(), sol.data());
   expect(ok && rel(sol, b) < 1e-7, "factory gmres+iluk", rel(sol, b));
   expect(factory.create("no-such-solver", A) == nullptr, "factory unknown name", 0);
+
+  // FGMRES with a right operator (mkl_solver.cpp:319-324): returns u with
+  // A R^-1 u = b; x = R^-1 u solves A x = b
+  {
+    auto R = std::make_shared<mkl_wrapper::incomplete_lu_k>();
+    R->set_level(0);
+    R->symbolic_factorize(&A);
+    R->numeric_factorize(&A);
+    mkl_wrapper::mkl_fgmres_solver fr(&A, nullptr, R);
+    fr.set_rel_tol(1e-11);
+    fr.set_restart_steps(30);
+    fr.set_max_iters(500);
+    std::vector<double> u(n, 0.0), xr(n, 0.0);
+    ok = fr.solve(rhs.data(), u.data());
+    R->solve(u.data(), xr.data());
+    expect(ok && rel(xr, b) < 1e-7, "mkl_fgmres_solver with right operator", rel(xr, b));
+  }
+
+  // symmetric scaling (mkl_sparse_mat.cpp:517-544): D A D with D from the
+  // row norms has unit-ish rows; the product follows the scaled values
+  {
+    mkl_wrapper::mkl_sparse_mat S(n, n, ai, aj, av);
+    auto d = S.rowwiseSqrtNorm();
+    S.DtAD(d);
+    std::vector<double> y(n), yref(n);
+    S.mult_vec(b.data(), y.data());
+    for (int i = 0; i < n; ++i) {
+      double acc = 0;
+      for (int q = ai[i]; q < ai[i + 1]; ++q) acc += av[q] * d[i] * d[aj[q]] * b[aj[q]];
+      yref[i] = acc;
+    }
+    expect(rel(y, yref) < 1e-13, "rowwiseSqrtNorm + DtAD product", rel(y, yref));
+  }
   std::printf("%d failure(s)\n", failures);
   return failures;
 }

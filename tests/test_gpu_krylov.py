@@ -155,6 +155,28 @@ def test_fgmres_mixed_ir(dev, prec):
     assert res["converged"] and res["rel_residual"] < 1e-10
 
 
+def test_fgmres_right_operator(dev):
+    """mkl_fgmres_solver(A, P, R) (mkl_solver.cpp:319-324): FGMRES on
+    A R^-1 u = b returns u; x = R^-1 u solves A x = b (checked against a
+    dense solve). R = ILU(0), P = ILU(1)."""
+    import torch
+    A = la.gen.powerlaw(1500, 8.0, 1.0, 200, seed=8)
+    R = la.Preconditioner(la.iluk(A, 0), "ilu")
+    P = la.Preconditioner(la.iluk(A, 1), "ilu")
+    b = torch.tensor(la.gen.uniform(A.n, 12), device=dev)
+    u = torch.zeros_like(b)
+    res = la.fgmres_right(la.DeviceMatrix(A), P, R, b, u, rel_tol=1e-12, restart=30,
+                          max_iters=500)
+    assert res["converged"] and res["rel_residual"] < 1e-12
+    x = torch.zeros_like(b)
+    R.apply(u, x)
+    D = orc.CSR(A.n, A.ai, A.aj, A.av).to_scipy().toarray()
+    xr = np.linalg.solve(D, b.cpu().numpy())
+    assert np.linalg.norm(x.cpu().numpy() - xr) <= 1e-9 * np.linalg.norm(xr)
+    with pytest.raises(la.LaError):
+        la.fgmres_right(la.DeviceMatrix(A), P, R, b, u, precision=la.PREC_MIXED_F32)
+
+
 def test_unknown_precision_rejected(dev):
     import torch
     A = la.gen.poisson2d(8)
