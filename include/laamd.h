@@ -46,12 +46,14 @@ typedef enum { LA_F64 = 0, LA_F32 = 1, LA_BF16 = 2 } la_dtype;
 
 /* sparse_mat_op/spmv.hpp: SerialSPMV/ParallelSPMV (:12-57) -> ROW,
  * SegSumSPMV (:59-185) -> SEGSUM, ALBUSSPMV (:186-300) -> ALBUS. AUTO picks
- * ROW or ALBUS from the row-length profile. */
+ * SELL when rows are near-uniform (little padding), else ALBUS. */
 typedef enum {
   LA_SPMV_ROW = 0,
   LA_SPMV_SEGSUM = 1,
   LA_SPMV_ALBUS = 2,
-  LA_SPMV_AUTO = 3
+  LA_SPMV_AUTO = 3,
+  LA_SPMV_SELL = 4 /* sliced ELL copy, one lane per row (no counterpart in the
+                      reference; AUTO picks it for near-uniform row lengths) */
 } la_spmv_alg;
 
 /* triangle_solve.hpp:147-208 OptimizedTriangularSolve modes: Barrier ->

@@ -272,6 +272,7 @@ void alloc_padded(DevBuf &b, la_dtype dt, int64_t cnt, hipStream_t s) {
 
 void MultiValues::set_from_device_f64(const double *d, int64_t cnt, hipStream_t s) {
   count = cnt;
+  ++version;
   if (!has[LA_F64]) {
     alloc_padded(v[LA_F64], LA_F64, cnt, s);
     has[LA_F64] = true;
@@ -284,6 +285,7 @@ void MultiValues::set_from_device_f64(const double *d, int64_t cnt, hipStream_t 
 
 void MultiValues::set_from_host(const double *h, int64_t cnt, hipStream_t s) {
   count = cnt;
+  ++version;
   if (!has[LA_F64]) {
     alloc_padded(v[LA_F64], LA_F64, cnt, s);
     has[LA_F64] = true;

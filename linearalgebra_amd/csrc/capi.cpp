@@ -322,8 +322,10 @@ int la_mat_info(la_mat_t A, int *rows, int *cols, int64_t *nnz) {
 int la_spmv_preprocess(la_mat_t A, la_spmv_alg alg) {
   LA_TRY
   LA_CHECK_ARG(A, "null handle");
+  LA_CHECK_ARG(alg >= LA_SPMV_ROW && alg <= LA_SPMV_SELL, "bad SpMV algorithm");
   if (alg == LA_SPMV_SEGSUM || alg == LA_SPMV_ALBUS || alg == LA_SPMV_AUTO)
     spmv_prepare_segments(*A->m);
+  if (alg == LA_SPMV_SELL || alg == LA_SPMV_AUTO) spmv_prepare_sell(*A->m);
   LA_CATCH
 }
 int la_spmv(la_mat_t A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, double alpha,

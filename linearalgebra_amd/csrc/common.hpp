@@ -103,6 +103,7 @@ struct MultiValues {
   DevBuf v[3];  // indexed by la_dtype
   bool has[3] = {false, false, false};
   int64_t count = 0;
+  uint64_t version = 0;  // bumped whenever the values change
   // uploads FP64 host values and converts every requested precision
   void set_from_host(const double *h, int64_t cnt, hipStream_t s);
   void set_from_device_f64(const double *d, int64_t cnt, hipStream_t s);

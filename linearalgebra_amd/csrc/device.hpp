@@ -23,6 +23,13 @@ struct Mat {
   bool seg_ready = false;
   int64_t nseg = 0;
   DevBuf seg_first_row, seg_head, seg_tail, seg_tail_row, seg_tail_end;
+  // sliced ELL copy (slices of 64 rows, column-major inside a slice): built
+  // on first use; values gathered from `vals` per precision when stale
+  int sell_state = 0;  // 0 not built, 1 built, -1 not worth it (too much padding)
+  int64_t sell_nslices = 0, sell_len = 0;
+  int sell_uniform_w = 0;  // > 0 when every slice has this width
+  DevBuf sell_ptr, sell_w, sell_col, sell_src, sell_val[3];
+  uint64_t sell_ver[3] = {0, 0, 0};
   // A^T, built on first transposed product
   std::unique_ptr<Mat> transposed;
   // host copy of the pattern, kept for the transpose and the value map
@@ -32,6 +39,8 @@ struct Mat {
 std::unique_ptr<Mat> make_mat(int rows, int cols, int base, const int *ai, const int *aj,
                               const double *av, hipStream_t s);
 void spmv_prepare_segments(Mat &A);
+// builds the sliced-ELL copy; false when the matrix is not suited to it
+bool spmv_prepare_sell(Mat &A);
 // rowwiseSqrtNorm / DtAD on the device matrix (d: device, rows entries)
 void mat_rowwise_sqrt_norm(Mat &A, double *d, hipStream_t s);
 void mat_dtad(Mat &A, const double *d, hipStream_t s);

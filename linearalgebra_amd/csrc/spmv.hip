@@ -11,6 +11,7 @@
 //    a second tiny kernel adds them in slice order (the ALBUS boundary fix-up
 //    of spmv.hpp:275-292, deterministic). Replaces SegSumSPMV and ALBUSSPMV.
 #include <algorithm>
+#include <climits>
 
 #include "dev_util.hpp"
 #include "device.hpp"
@@ -282,17 +283,203 @@ void launch_seg(Mat &M, const T *av, const V *x, V *y, const V *z, A alpha, A be
   LA_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ SELL
+// Sliced ELL: slices of 64 consecutive rows, each stored column-major with
+// the slice's longest row as its width, so lane l of a wavefront walks row
+// 64s+l left to right (SerialSPMV's order) with every load coalesced across
+// the wavefront. Padding entries carry column -1 and add nothing (a
+// non-finite x entry elsewhere cannot leak in); the copy's values are
+// gathered from the CSR values of the same precision.
+constexpr double kSellMaxFill = 1.15;  // padded entries / nnz allowed
+constexpr int kSellMaxWidth = 64;      // longest row allowed
+
+// W consecutive slice columns of one lane: all loads issued before any
+// product (the slice width is uniform across the wavefront)
+// (NTC: the column indices with non-temporal loads too; measured on MI355X,
+// narrow slices -- width <= 8, the 7-point stencil -- run faster with
+// ordinary column loads, wide ones -- the 27-point stencil -- with
+// non-temporal ones)
+template <int W, bool NTC, typename T, typename V, typename A>
+__device__ __forceinline__ A sell_block(A acc, int k0, int64_t base,
+                                        const int *__restrict__ scol,
+                                        const T *__restrict__ sval, const V *__restrict__ x) {
+  using LT = decltype(ldm(sval, int64_t(0)));  // bf16 loads widen to float
+  int c[W];
+  LT v[W];
+#pragma unroll
+  for (int e = 0; e < W; ++e) {
+    const int64_t q = base + static_cast<int64_t>(k0 + e) * 64;
+    c[e] = NTC ? __builtin_nontemporal_load(scol + q) : scol[q];
+    v[e] = ldm(sval, q);
+  }
+  V xv[W];
+#pragma unroll
+  for (int e = 0; e < W; ++e) xv[e] = ldx(x, c[e] < 0 ? 0 : c[e]);
+#pragma unroll
+  for (int e = 0; e < W; ++e) {
+    const A p = A(v[e]) * A(xv[e]);
+    acc = c[e] >= 0 ? acc + p : acc;  // padding (column -1) contributes nothing
+  }
+  return acc;
+}
+
+template <typename T, typename V, typename A>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell(
+    int n, int64_t nslices, int uw, const int64_t *__restrict__ sp,
+    const int *__restrict__ sw, const int *__restrict__ scol, const T *__restrict__ sval,
+    const V *__restrict__ x, V *y, const V *z, A alpha, A beta) {
+  const int64_t sl = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * (kBlock / 64) +
+                     (threadIdx.x >> 6);
+  if (sl >= nslices) return;
+  const int lane = threadIdx.x & 63;
+  const int row = static_cast<int>(sl * 64 + lane);
+  // uw > 0: every slice has that width (no slice table to read)
+  const int64_t base = (uw > 0 ? sl * 64 * uw : sp[sl]) + lane;
+  const int w = uw > 0 ? uw : sw[sl];
+  A acc = 0;
+#define LA_SB(WW, NT) acc = sell_block<WW, NT, T, V, A>(acc, k, base, scol, sval, x)
+  int k = 0;
+  if (w <= 8) {
+    switch (w) {  // the whole row in one go
+      case 8: LA_SB(8, false); break;
+      case 7: LA_SB(7, false); break;
+      case 6: LA_SB(6, false); break;
+      case 5: LA_SB(5, false); break;
+      case 4: LA_SB(4, false); break;
+      case 3: LA_SB(3, false); break;
+      case 2: LA_SB(2, false); break;
+      case 1: LA_SB(1, false); break;
+      default: break;
+    }
+  } else {
+    for (; k + 16 <= w; k += 16) LA_SB(16, true);
+    if (k + 8 <= w) {
+      LA_SB(8, true);
+      k += 8;
+    }
+    switch (w - k) {  // the rest: 0..7 columns
+      case 7: LA_SB(7, true); break;
+      case 6: LA_SB(6, true); break;
+      case 5: LA_SB(5, true); break;
+      case 4: LA_SB(4, true); break;
+      case 3: LA_SB(3, true); break;
+      case 2: LA_SB(2, true); break;
+      case 1: LA_SB(1, true); break;
+      default: break;
+    }
+  }
+#undef LA_SB
+  if (row < n) {
+    A o = alpha * acc;
+    if (beta != A(0)) o += beta * A(z[row]);
+    y[row] = V(o);
+  }
+}
+
+template <typename T>
+__global__ void k_sell_gather(int64_t m, const int *__restrict__ src, const T *__restrict__ v,
+                              T *out) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < m) {
+    const int q = src[t];
+    out[t] = q >= 0 ? v[q] : T();
+  }
+}
+
+// builds the slice structure once; false when padding would cost too much
+bool prepare_sell(Mat &M) {
+  if (M.sell_state) return M.sell_state > 0;
+  const int n = M.rows;
+  const int64_t ns = (n + 63) / 64;
+  std::vector<int64_t> sp(static_cast<size_t>(ns) + 1, 0);
+  std::vector<int> sw(static_cast<size_t>(std::max<int64_t>(ns, 1)), 0);
+  for (int64_t s = 0; s < ns; ++s) {
+    int w = 0;
+    for (int r = static_cast<int>(s * 64); r < std::min<int64_t>(n, (s + 1) * 64); ++r)
+      w = std::max(w, M.h_ai[r + 1] - M.h_ai[r]);
+    sw[s] = w;
+    sp[s + 1] = sp[s] + 64 * static_cast<int64_t>(w);
+  }
+  const int64_t len = sp[ns];
+  const int maxw = ns ? *std::max_element(sw.begin(), sw.begin() + ns) : 0;
+  if (M.nnz == 0 || static_cast<double>(len) > kSellMaxFill * static_cast<double>(M.nnz) ||
+      maxw > kSellMaxWidth || len >= INT32_MAX) {
+    M.sell_state = -1;
+    return false;
+  }
+  std::vector<int> col(static_cast<size_t>(len), -1), src(static_cast<size_t>(len), -1);
+  for (int64_t s = 0; s < ns; ++s)
+    for (int l = 0; l < 64; ++l) {
+      const int64_t r = s * 64 + l;
+      if (r >= n) break;
+      const int b = M.h_ai[r], rl = M.h_ai[r + 1] - b;
+      for (int k = 0; k < rl; ++k) {
+        const int64_t p = sp[s] + static_cast<int64_t>(k) * 64 + l;
+        col[p] = M.h_aj[b + k];
+        src[p] = b + k;
+      }
+    }
+  M.sell_nslices = ns;
+  M.sell_len = len;
+  M.sell_uniform_w = (ns && std::all_of(sw.begin(), sw.begin() + ns, [&](int w) { return w == sw[0]; }))
+                         ? sw[0] : 0;
+  M.sell_ptr.alloc(sizeof(int64_t) * sp.size());
+  M.sell_ptr.upload(sp.data(), sizeof(int64_t) * sp.size(), M.stream);
+  M.sell_w.alloc(sizeof(int) * sw.size());
+  M.sell_w.upload(sw.data(), sizeof(int) * sw.size(), M.stream);
+  M.sell_col.alloc(sizeof(int) * col.size());
+  M.sell_col.upload(col.data(), sizeof(int) * col.size(), M.stream);
+  M.sell_src.alloc(sizeof(int) * src.size());
+  M.sell_src.upload(src.data(), sizeof(int) * src.size(), M.stream);
+  LA_HIP(hipStreamSynchronize(M.stream));
+  M.sell_state = 1;
+  return true;
+}
+
+template <typename T>
+const T *sell_values(Mat &M, la_dtype dt, const T *av, hipStream_t s) {
+  if (M.sell_ver[dt] != M.vals.version || !M.sell_val[dt].p) {
+    if (!M.sell_val[dt].p) M.sell_val[dt].alloc(sizeof(T) * static_cast<size_t>(M.sell_len));
+    hipLaunchKernelGGL(k_sell_gather<T>, dim3(static_cast<unsigned>((M.sell_len + 255) / 256)),
+                       dim3(256), 0, s, M.sell_len, M.sell_src.as<int>(), av,
+                       M.sell_val[dt].as<T>());
+    LA_HIP(hipGetLastError());
+    M.sell_ver[dt] = M.vals.version;
+  }
+  return M.sell_val[dt].as<T>();
+}
+
+template <typename T> struct DtOf;
+template <> struct DtOf<double> { static constexpr la_dtype v = LA_F64; };
+template <> struct DtOf<float> { static constexpr la_dtype v = LA_F32; };
+template <> struct DtOf<bf16> { static constexpr la_dtype v = LA_BF16; };
+
+template <typename T, typename V, typename A>
+void launch_sell(Mat &M, const T *av, const V *x, V *y, const V *z, A alpha, A beta,
+                 hipStream_t s) {
+  const T *sv = sell_values<T>(M, DtOf<T>::v, av, s);
+  const unsigned grid = static_cast<unsigned>((M.sell_nslices + kBlock / 64 - 1) / (kBlock / 64));
+  hipLaunchKernelGGL((k_spmv_sell<T, V, A>), dim3(grid), dim3(kBlock), 0, s, M.rows,
+                     M.sell_nslices, M.sell_uniform_w, M.sell_ptr.as<int64_t>(),
+                     M.sell_w.as<int>(),
+                     M.sell_col.as<int>(), sv, x, y, z, alpha, beta);
+  LA_HIP(hipGetLastError());
+}
+
 template <typename T, typename V, typename A>
 void dispatch_alg(Mat &M, la_spmv_alg alg, const void *av, const void *x, void *y,
                   const void *z, double alpha, double beta, hipStream_t s) {
-  if (alg == LA_SPMV_AUTO) alg = LA_SPMV_ALBUS;
   if (M.nnz == 0 || M.cols == 0) alg = LA_SPMV_ROW;  // nothing to gather
+  if (alg == LA_SPMV_AUTO) alg = prepare_sell(M) ? LA_SPMV_SELL : LA_SPMV_ALBUS;
+  if (alg == LA_SPMV_SELL && !prepare_sell(M)) alg = LA_SPMV_ALBUS;
   const T *a = static_cast<const T *>(av);
   const V *xx = static_cast<const V *>(x);
   V *yy = static_cast<V *>(y);
   const V *zz = static_cast<const V *>(z);
   if (alg == LA_SPMV_ROW)
     launch_row<T, V, A>(M, a, xx, yy, zz, A(alpha), A(beta), s);
+  else if (alg == LA_SPMV_SELL)
+    launch_sell<T, V, A>(M, a, xx, yy, zz, A(alpha), A(beta), s);
   else
     launch_seg<T, V, A>(M, a, xx, yy, zz, A(alpha), A(beta), s);
 }
@@ -339,6 +526,8 @@ std::unique_ptr<Mat> make_mat(int rows, int cols, int base, const int *ai, const
   return M;
 }
 
+bool spmv_prepare_sell(Mat &A) { return prepare_sell(A); }
+
 void spmv_prepare_segments(Mat &A) {
   const int64_t nseg = std::max<int64_t>(1, (A.nnz + kSegNnz - 1) / kSegNnz);
   A.nseg = nseg;
@@ -359,7 +548,7 @@ void spmv(Mat &A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, dou
   LA_CHECK_ARG(val >= LA_F64 && val <= LA_BF16, "bad value dtype");
   LA_CHECK_ARG(vec == LA_F64 || vec == LA_F32, "vectors must be FP64 or FP32");
   LA_CHECK_ARG(acc == LA_F64 || acc == LA_F32, "accumulation must be FP64 or FP32");
-  LA_CHECK_ARG(alg >= LA_SPMV_ROW && alg <= LA_SPMV_AUTO, "bad SpMV algorithm");
+  LA_CHECK_ARG(alg >= LA_SPMV_ROW && alg <= LA_SPMV_SELL, "bad SpMV algorithm");
   LA_CHECK_ARG(A.rows == 0 || (x && y), "null vector");
   if (A.rows == 0) return;
   if (!A.vals.has[val]) A.vals.add(val, s);

@@ -64,7 +64,8 @@ def _absbound(A, x):
 
 
 @pytest.mark.parametrize("name,make", _cases(), ids=[c[0] for c in _cases()])
-@pytest.mark.parametrize("alg", [la.SPMV_ROW, la.SPMV_SEGSUM, la.SPMV_ALBUS, la.SPMV_AUTO])
+@pytest.mark.parametrize("alg", [la.SPMV_ROW, la.SPMV_SEGSUM, la.SPMV_ALBUS, la.SPMV_AUTO,
+                                 la.SPMV_SELL])
 def test_spmv_f64(dev, name, make, alg):
     import torch
     A = make()
@@ -100,7 +101,7 @@ def test_spmv_beta_zero_ignores_y(dev, alg):
 
 @pytest.mark.parametrize("name,make", _cases()[:7], ids=[c[0] for c in _cases()[:7]])
 @pytest.mark.parametrize("val,bound", [(la.F32, 2e-6), (la.BF16, 2e-2)])
-@pytest.mark.parametrize("alg", [la.SPMV_ROW, la.SPMV_ALBUS])
+@pytest.mark.parametrize("alg", [la.SPMV_ROW, la.SPMV_ALBUS, la.SPMV_SELL])
 def test_spmv_lowprec(dev, name, make, val, bound, alg):
     import torch
     A = make()
@@ -155,6 +156,29 @@ def test_value_grad(dev):
     rows = np.repeat(np.arange(A.n), np.diff(A.ai))
     ref = -lam[rows] * x[A.aj - A.base]
     np.testing.assert_array_equal(g.cpu().numpy(), ref)
+
+
+def test_sell_follows_new_values(dev):
+    """The sliced-ELL copy is re-gathered after the CSR values change, and
+    a NaN in x outside a row's columns stays out of that row."""
+    import torch
+    A = _o(la.gen.laplace3d(20))
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    x = la.gen.uniform(A.n, 5)
+    xt = torch.tensor(x, device=dev)
+    y = torch.zeros_like(xt)
+    dA.spmv(xt, y, alg=la.SPMV_SELL)
+    assert np.all(np.abs(y.cpu().numpy() - orc.spmv(A, x)) <= 1e-13 * _absbound(A, x))
+    v2 = np.asarray(A.av) * -2.5
+    dA.set_values(torch.tensor(v2, device=dev))
+    dA.spmv(xt, y, alg=la.SPMV_SELL)
+    A2 = orc.CSR(A.n, A.ai, A.aj, v2)
+    assert np.all(np.abs(y.cpu().numpy() - orc.spmv(A2, x)) <= 1e-13 * _absbound(A2, x))
+    # row 0 of the 7-point operator touches columns 0, 1, 20, 400 only
+    xn = x.copy()
+    xn[A.n - 1] = np.nan
+    dA.spmv(torch.tensor(xn, device=dev), y, alg=la.SPMV_SELL)
+    assert np.isfinite(y.cpu().numpy()[0])
 
 
 def test_spmv_repeat_deterministic(dev):
