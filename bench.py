@@ -6,7 +6,9 @@ int32 indices, RCM-reordered on the host.
 
   * value: FP64 CSR SpMV y = A x throughput in GB/s of algorithmic bytes
     B = nnz*(8+4) + (n+1)*4 + n*8 (x, counted once) + n*8 (y), K timed
-    launches. The matrix is held in three copies used in rotation, so a
+    launches of the kernel the default (AUTO) request runs -- the sliced-ELL
+    kernel for this matrix; the padding its slices carry is read but not
+    counted in B. The matrix is held in three copies used in rotation, so a
     launch never finds its matrix in the 256 MB Infinity Cache: each timed
     launch streams A from HBM.
   * pcg: IC(0)-PCG on b ~ U[-1,1) (seeded) to a 1e-10 true relative residual,
@@ -39,6 +41,15 @@ sys.path.insert(0, ROOT)
 METRIC = ("SpMV GB/s (%HBM peak); PCG time to 1e-10: mixed FP32/BF16 vs FP64, "
           "1 GPU")
 HBM_PEAK_GBS = 8000.0
+# SPMV_* codes (ROW 0, SEGSUM 1, ALBUS 2, AUTO 3, SELL 4)
+ALG_NAMES = {0: "row", 1: "segsum", 2: "albus", 3: "auto", 4: "sell"}
+ALG_DESC = {0: "one wavefront per row", 2: "nnz-balanced segment kernel, 512 nnz per "
+            "wavefront, + boundary-row fix-up",
+            4: "sliced ELL, 64-row column-major slices, one lane per row"}
+ALG_KERNEL = {0: "k_spmv_row<G,double,double,double>",
+              2: "k_spmv_seg<double,double,double> + k_spmv_seg_fix (one SpMV; duration "
+                 "of the pair)",
+              4: "k_spmv_sell<double,double,double>"}
 
 
 def log(*a):
@@ -218,9 +229,10 @@ def bench_c3(la, torch, dev, stream, args):
     rl = np.diff(A.ai)
     mats = [la.DeviceMatrix(A, stream=stream) for _ in range(2)]
     tab = _spmv_table(la, torch, A, mats, dev, stream,
-                      [("albus", la.SPMV_ALBUS), ("row", la.SPMV_ROW)],
+                      [("auto", la.SPMV_AUTO), ("albus", la.SPMV_ALBUS), ("row", la.SPMV_ROW)],
                       [(la.F64, "f64"), (la.F32, "f32"), (la.BF16, "bf16")], args.steps)
     print(json.dumps({"config": "c3 power-law n=4M", "n": A.n, "nnz": A.nnz,
+                      "auto_runs": ALG_NAMES[mats[0].resolve(la.SPMV_AUTO)],
                       "max_row": int(rl.max()), "gen_s": prep, "spmv": tab,
                       "note": "f32/bf16: values in that type, FP32 vectors and accumulation"}))
 
@@ -234,7 +246,8 @@ def bench_c4(la, torch, dev, stream, args):
     A = la.gen.aniso27(grid)
     t_gen = time.time() - t0
     mats = [la.DeviceMatrix(A, stream=stream)]
-    tab = _spmv_table(la, torch, A, mats, dev, stream, [("albus", la.SPMV_ALBUS)],
+    tab = _spmv_table(la, torch, A, mats, dev, stream,
+                      [("auto", la.SPMV_AUTO), ("albus", la.SPMV_ALBUS)],
                       [(la.F64, "f64"), (la.F32, "f32"), (la.BF16, "bf16")], 10)
     t0 = time.time()
     fai, faj, fu = la.iluk_symbolic(A, 1)
@@ -295,6 +308,7 @@ def bench_c4(la, torch, dev, stream, args):
         torch.cuda.synchronize()
         ap[vn] = round((time.perf_counter() - t0) / 5 * 1e3, 3)
     out = {"config": "c4 aniso27 %d^3" % grid, "n": A.n, "nnz": A.nnz, "spmv": tab,
+           "auto_runs": ALG_NAMES[mats[0].resolve(la.SPMV_AUTO)],
            "ilu1": {"nnz": int(len(faj)), "levels_L_U": P.levels(), "gen_s": round(t_gen, 2),
                     "symbolic_s": round(t_sym, 2), "numeric_s": round(t_num, 2),
                     "analysis_upload_s": round(t_an, 2), "apply_ms": ap,
@@ -349,7 +363,7 @@ def main():
 
     if args.pmc_probe:
         for i in range(6):
-            mats[i % 3].spmv(x, y, alg=la.SPMV_ALBUS)
+            mats[i % 3].spmv(x, y, alg=la.SPMV_AUTO)
         big = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
         dst = torch.empty_like(big)
         dst.copy_(big)
@@ -359,13 +373,14 @@ def main():
 
     # parity check of the timed kernel against the FP64 oracle (cheap)
     from oracle import oracle as orc
-    mats[0].spmv(x, y, alg=la.SPMV_ALBUS)
+    chosen = mats[0].resolve(la.SPMV_AUTO)
+    mats[0].spmv(x, y, alg=la.SPMV_AUTO)
     ref = orc.spmv(orc.CSR(n, A.ai, A.aj, A.av), x.cpu().numpy())
     rel = float(np.linalg.norm(y.cpu().numpy() - ref) / np.linalg.norm(ref))
     if rel > 1e-12:
         raise SystemExit("SpMV parity failure: rel err %.3e" % rel)
 
-    durs, wall_ms = time_spmv(la, torch, mats, x, y, args.steps, args.warmup, la.SPMV_ALBUS,
+    durs, wall_ms = time_spmv(la, torch, mats, x, y, args.steps, args.warmup, la.SPMV_AUTO,
                               la.F64, la.F64, stream)
     B = spmv_bytes(n, nnz)
     avg_ms = float(np.mean(durs))
@@ -373,7 +388,8 @@ def main():
     gbs_wall = B * args.steps / (wall_ms * 1e-3) / 1e9
     extra = {}
     # the other SpMV forms on the same matrix, for the record
-    for name, alg, val, acc in [("row_f64", la.SPMV_ROW, la.F64, la.F64)]:
+    for name, alg, val, acc in [("albus_f64", la.SPMV_ALBUS, la.F64, la.F64),
+                                ("row_f64", la.SPMV_ROW, la.F64, la.F64)]:
         d2, _ = time_spmv(la, torch, mats, x, y, 20, 3, alg, val, acc, stream)
         extra[name] = round(B / (np.mean(d2) * 1e-3) / 1e9, 1)
     x32 = x.float()
@@ -381,14 +397,15 @@ def main():
     for dt, sv, name in [(la.F32, 4, "f32"), (la.BF16, 2, "bf16")]:
         for m in mats:
             m.add_dtype(dt)
-        d2, _ = time_spmv(la, torch, mats, x32, y32, 20, 3, la.SPMV_ALBUS, dt, la.F32, stream)
+        d2, _ = time_spmv(la, torch, mats, x32, y32, 20, 3, la.SPMV_AUTO, dt, la.F32, stream)
         Bl = spmv_bytes(n, nnz, s_v=sv, s_x=4, s_y=4)
-        extra["albus_%s_vals_f32_vecs" % name] = {
+        extra["%s_%s_vals_f32_vecs" % (ALG_NAMES[chosen], name)] = {
             "GBs": round(Bl / (np.mean(d2) * 1e-3) / 1e9, 1),
             "us": round(float(np.mean(d2)) * 1e3, 2)}
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_spmv_albus_f64_laplace3d_128.json")
+    tpath = os.path.join(ROOT, "profiles",
+                         "traffic_spmv_%s_f64_laplace3d_128.json" % ALG_NAMES[chosen])
     if os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
@@ -428,8 +445,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "3-D 7-point Laplacian 128^3, FP64 CSR int32, RCM-ordered; "
-                               "y = A x (nnz-balanced segment kernel, 512 nnz per "
-                               "wavefront, + boundary-row fix-up)",
+                               "y = A x (%s)" % ALG_DESC[chosen],
                    "n": n, "nnz": nnz, "bytes_per_spmv": B,
                    "pct_hbm_peak": round(100 * gbs_kernel / HBM_PEAK_GBS, 1),
                    "wall_GBs": round(gbs_wall, 1), "host_prep": prep,
@@ -437,8 +453,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs_kernel, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs_kernel / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "k_spmv_seg<double,double,double> + k_spmv_seg_fix "
-                               "(one SpMV; duration of the pair)",
+                     "kernel": ALG_KERNEL[chosen],
                      "avg_launch_us": round(avg_ms * 1e3, 3)},
         "cpu_baseline": cpu,
         "pcg": pcg,

@@ -173,6 +173,11 @@ int la_mat_info(la_mat_t A, int *rows, int *cols, int64_t *nnz);
 /* SegSumSPMV/ALBUSSPMV::preprocess (sparse_mat_op/spmv.hpp:68-77,195-228):
  * nnz-balanced segment table. Called implicitly on first use. */
 int la_spmv_preprocess(la_mat_t A, la_spmv_alg alg);
+/* The kernel family a request runs on for this matrix: AUTO and SELL give
+ * SELL when the sliced-ELL padding stays small, else ALBUS; SEGSUM runs on
+ * ALBUS's kernels; an empty matrix runs ROW. Builds what the choice needs.
+ * No counterpart in the reference (its algorithm is a template argument). */
+int la_spmv_resolve(la_mat_t A, la_spmv_alg alg, int *chosen);
 /* y = alpha*op(A)*x + beta*y (device pointers), the functor call
  * operator()(size, base, ai, aj, av, b, x, alpha, beta) of
  * sparse_mat_op/spmv.hpp:16-29 with b -> x and x -> y, and

@@ -125,6 +125,12 @@ class DeviceMatrix:
     def preprocess(self, alg=SPMV_ALBUS):
         check(lib().la_spmv_preprocess(self.h, alg))
 
+    def resolve(self, alg=SPMV_AUTO):
+        """The kernel family `alg` runs on for this matrix (SPMV_* code)."""
+        out = C.c_int(0)
+        check(lib().la_spmv_resolve(self.h, alg, C.byref(out)))
+        return out.value
+
     def spmv(self, x, y, alpha=1.0, beta=0.0, alg=SPMV_AUTO, val=None, acc=None, z=None):
         """y = alpha*A*x + beta*(z if given else y)."""
         vec = _vec_code(x)

@@ -328,6 +328,13 @@ int la_spmv_preprocess(la_mat_t A, la_spmv_alg alg) {
   if (alg == LA_SPMV_SELL || alg == LA_SPMV_AUTO) spmv_prepare_sell(*A->m);
   LA_CATCH
 }
+int la_spmv_resolve(la_mat_t A, la_spmv_alg alg, int *chosen) {
+  LA_TRY
+  LA_CHECK_ARG(A && chosen, "bad argument");
+  LA_CHECK_ARG(alg >= LA_SPMV_ROW && alg <= LA_SPMV_SELL, "bad SpMV algorithm");
+  *chosen = static_cast<int>(spmv_resolve(*A->m, alg));
+  LA_CATCH
+}
 int la_spmv(la_mat_t A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, double alpha,
             const void *x, double beta, void *y) {
   LA_TRY

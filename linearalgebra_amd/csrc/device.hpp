@@ -41,6 +41,8 @@ std::unique_ptr<Mat> make_mat(int rows, int cols, int base, const int *ai, const
 void spmv_prepare_segments(Mat &A);
 // builds the sliced-ELL copy; false when the matrix is not suited to it
 bool spmv_prepare_sell(Mat &A);
+// the kernel family a request runs on (SEGSUM shares ALBUS's kernels)
+la_spmv_alg spmv_resolve(Mat &M, la_spmv_alg alg);
 // rowwiseSqrtNorm / DtAD on the device matrix (d: device, rows entries)
 void mat_rowwise_sqrt_norm(Mat &A, double *d, hipStream_t s);
 void mat_dtad(Mat &A, const double *d, hipStream_t s);

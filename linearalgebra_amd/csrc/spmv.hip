@@ -528,6 +528,13 @@ std::unique_ptr<Mat> make_mat(int rows, int cols, int base, const int *ai, const
 
 bool spmv_prepare_sell(Mat &A) { return prepare_sell(A); }
 
+la_spmv_alg spmv_resolve(Mat &M, la_spmv_alg alg) {
+  if (M.nnz == 0 || M.cols == 0) return LA_SPMV_ROW;
+  if (alg == LA_SPMV_AUTO || alg == LA_SPMV_SELL)
+    return prepare_sell(M) ? LA_SPMV_SELL : LA_SPMV_ALBUS;
+  return alg == LA_SPMV_SEGSUM ? LA_SPMV_ALBUS : alg;
+}
+
 void spmv_prepare_segments(Mat &A) {
   const int64_t nseg = std::max<int64_t>(1, (A.nnz + kSegNnz - 1) / kSegNnz);
   A.nseg = nseg;

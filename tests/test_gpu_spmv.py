@@ -195,3 +195,17 @@ def test_spmv_repeat_deterministic(dev):
         outs.append(y.cpu().numpy())
     for o in outs[1:]:
         np.testing.assert_array_equal(o, outs[0])
+
+
+def test_resolve_names_the_kernel(dev):
+    """AUTO runs sliced ELL on a stencil (small padding) and the segment
+    kernels on long, ragged rows; an empty matrix runs the row kernel."""
+    st = la.DeviceMatrix(la.gen.laplace3d(12))
+    assert st.resolve(la.SPMV_AUTO) == la.SPMV_SELL
+    assert st.resolve(la.SPMV_SEGSUM) == la.SPMV_ALBUS
+    assert st.resolve(la.SPMV_ROW) == la.SPMV_ROW
+    pl = la.DeviceMatrix(la.gen.powerlaw(20000, 16.0, 2.0, 6000, seed=5))
+    assert pl.resolve(la.SPMV_AUTO) == la.SPMV_ALBUS
+    assert pl.resolve(la.SPMV_SELL) == la.SPMV_ALBUS
+    em = la.DeviceMatrix(as_lib_csr(orc.CSR(7, np.zeros(8), [], [])))
+    assert em.resolve(la.SPMV_AUTO) == la.SPMV_ROW
