@@ -58,8 +58,10 @@ typedef enum {
 
 /* triangle_solve.hpp:147-208 OptimizedTriangularSolve modes: Barrier ->
  * LEVEL (one launch per level), NoBarrier -> SYNCFREE (one launch, per-row
- * readiness). */
-typedef enum { LA_TRSV_LEVEL = 0, LA_TRSV_SYNCFREE = 1 } la_trsv_mode;
+ * readiness). JACOBI has no counterpart in the reference: k Jacobi sweeps
+ * on the triangle (k set per plan, default 3), exact for k >= levels - 1,
+ * otherwise an approximate solve for use as a preconditioner. */
+typedef enum { LA_TRSV_LEVEL = 0, LA_TRSV_SYNCFREE = 1, LA_TRSV_JACOBI = 2 } la_trsv_mode;
 
 typedef enum { LA_LOWER = 0, LA_UPPER = 1 } la_uplo;
 
@@ -215,6 +217,9 @@ int la_trsv_destroy(la_trsv_t T);
 int la_trsv_add_dtype(la_trsv_t T, la_dtype dt);
 int la_trsv_set_stream(la_trsv_t T, void *hip_stream);
 int la_trsv_info(la_trsv_t T, int *n, int *nlevels, int64_t *nnz);
+/* Sweep count of LA_TRSV_JACOBI solves with this plan (0 = diagonal scaling
+ * only); no counterpart in the reference. */
+int la_trsv_set_sweeps(la_trsv_t T, int sweeps);
 /* OptimizedTriangularSolve::operator()(b, x) (triangle_solve.tpp:115-125);
  * b and x are distinct device vectors. */
 int la_trsv_solve(la_trsv_t T, la_trsv_mode mode, la_dtype val, la_dtype vec,
@@ -262,6 +267,9 @@ int la_prec_apply(la_prec_t P, la_trsv_mode mode, la_dtype val, la_dtype vec,
 int la_prec_apply_transpose(la_prec_t P, la_trsv_mode mode, la_dtype val,
                             la_dtype vec, const void *r, void *z);
 int la_prec_info(la_prec_t P, int *levels_lower, int *levels_upper);
+/* Sweep count of LA_TRSV_JACOBI applies, both triangles (default 3); no
+ * counterpart in the reference. */
+int la_prec_set_sweeps(la_prec_t P, int sweeps);
 
 /* ====================== fused BLAS-1 (FP64 accumulation) =============== */
 int la_dot(int64_t n, la_dtype vec, const void *x, const void *y,
@@ -299,6 +307,12 @@ typedef struct {
                           step stops earlier once that reaches rel_tol/abs_tol, margin 4) */
   int max_outer;       /* mixed: refinement steps                          */
   int print_level;     /* set_print_level: 1 prints each residual          */
+  double reliable_delta; /* mixed PCG: > 0 (default 0.1) runs reliable
+                          updates -- one FP32 iteration whose residual is
+                          replaced by the FP64 true residual (and x updated
+                          in FP64) when it has fallen by this factor; <= 0
+                          runs restarted refinement (inner_tol, max_outer).
+                          No counterpart in the reference.                  */
 } la_solver_opts;
 
 typedef struct {

@@ -515,6 +515,8 @@ class device_preconditioner : public mkl_sparse_mat {
   la_prec_t prec_handle() const override { return P_.get(); }
   void set_trsv_mode(la_trsv_mode m) { trsv_mode_ = m; }
   la_trsv_mode trsv_mode() const { return trsv_mode_; }
+  // sweeps per triangle of LA_TRSV_JACOBI applies (an extension of this build)
+  void set_sweeps(int k) { detail::check(la_prec_set_sweeps(P_.get(), k)); }
 
  protected:
   void upload_factor(bool ic) {
@@ -642,6 +644,7 @@ class mkl_iterative_solver : public mkl_solver {
   void set_precision(la_precision p) { _precision = p; }
   void set_stop_rule(la_stop_rule r) { _stop = r; }
   void set_inner_tol(double t) { _inner_tol = t; }
+  void set_reliable_delta(double d) { _reliable_delta = d; }
   const la_solver_result &last_result() const { return _res; }
 
  protected:
@@ -654,6 +657,7 @@ class mkl_iterative_solver : public mkl_solver {
     o.precision = _precision;
     o.stop_rule = _stop;
     o.inner_tol = _inner_tol;
+    o.reliable_delta = _reliable_delta;
     o.print_level = _print_level;
     if (auto dp = dynamic_cast<const device_preconditioner *>(prec)) o.trsv_mode = dp->trsv_mode();
     (void)P;
@@ -665,6 +669,7 @@ class mkl_iterative_solver : public mkl_solver {
   la_precision _precision = LA_PREC_FP64;
   la_stop_rule _stop = LA_STOP_TRUE_RESIDUAL;
   double _inner_tol = 1e-5;
+  double _reliable_delta = 0.1;
   la_solver_result _res{};
 };
 

@@ -17,7 +17,7 @@ from . import _lib
 from ._lib import (BF16, F32, F64, LOWER, PREC_FP64, PREC_MIXED_BF16,
                    PREC_MIXED_BF16_F32P, PREC_MIXED_F32, SPMV_ALBUS, SPMV_AUTO, SPMV_ROW, SPMV_SELL,
                    SPMV_SEGSUM, STOP_RECURRENCE, STOP_TRUE_RESIDUAL,
-                   TRSV_LEVEL, TRSV_SYNCFREE, UPPER, LaError, SolverOpts,
+                   TRSV_JACOBI, TRSV_LEVEL, TRSV_SYNCFREE, UPPER, LaError, SolverOpts,
                    SolverResult, check)
 
 __all__ = [
@@ -194,6 +194,11 @@ class TriangularSolve:
     def add_dtype(self, dt):
         check(lib().la_trsv_add_dtype(self.h, dt))
 
+    def set_sweeps(self, k):
+        """Sweeps of TRSV_JACOBI solves (la_trsv_set_sweeps)."""
+        check(lib().la_trsv_set_sweeps(self.h, int(k)))
+        return self
+
     def solve(self, b, x, mode=TRSV_SYNCFREE, val=None):
         vec = _vec_code(b)
         val = vec if val is None else val
@@ -268,6 +273,11 @@ class Preconditioner:
 
     def add_dtype(self, dt):
         check(lib().la_prec_add_dtype(self.h, dt))
+
+    def set_sweeps(self, k):
+        """Sweeps per triangle of TRSV_JACOBI applies (la_prec_set_sweeps)."""
+        check(lib().la_prec_set_sweeps(self.h, int(k)))
+        return self
 
     def set_stream(self, stream):
         ptr = stream if isinstance(stream, int) else stream.cuda_stream

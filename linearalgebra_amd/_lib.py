@@ -13,7 +13,7 @@ LA_OK, LA_ERR_ARG, LA_ERR_HIP, LA_ERR_ALLOC = 0, 1, 2, 3
 LA_ERR_NOT_CONVERGED, LA_ERR_NUMERIC, LA_ERR_STATE, LA_ERR_TIMEOUT = 4, 5, 6, 7
 F64, F32, BF16 = 0, 1, 2
 SPMV_ROW, SPMV_SEGSUM, SPMV_ALBUS, SPMV_AUTO, SPMV_SELL = 0, 1, 2, 3, 4
-TRSV_LEVEL, TRSV_SYNCFREE = 0, 1
+TRSV_LEVEL, TRSV_SYNCFREE, TRSV_JACOBI = 0, 1, 2
 LOWER, UPPER = 0, 1
 PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16, PREC_MIXED_BF16_F32P = 0, 1, 2, 3
 STOP_TRUE_RESIDUAL, STOP_RECURRENCE = 0, 1
@@ -31,7 +31,7 @@ class SolverOpts(C.Structure):
                 ("stop_rule", C.c_int), ("precision", C.c_int),
                 ("trsv_mode", C.c_int), ("spmv_alg", C.c_int),
                 ("inner_tol", C.c_double), ("max_outer", C.c_int),
-                ("print_level", C.c_int)]
+                ("print_level", C.c_int), ("reliable_delta", C.c_double)]
 
 
 class SolverResult(C.Structure):
@@ -89,6 +89,7 @@ _SIGS = {
     "la_trsv_add_dtype": [_vp, _i],
     "la_trsv_set_stream": [_vp, _vp],
     "la_trsv_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i64)],
+    "la_trsv_set_sweeps": [_vp, _i],
     "la_trsv_solve": [_vp, _i, _i, _i, _vp, _vp],
     "la_trsv_solve_transpose": [_vp, _i, _i, _i, _vp, _vp],
     "la_prec_create_ilu": [_i, _i, _vp, _vp, _vp, C.POINTER(_vp)],
@@ -103,6 +104,7 @@ _SIGS = {
     "la_prec_apply": [_vp, _i, _i, _i, _vp, _vp],
     "la_prec_apply_transpose": [_vp, _i, _i, _i, _vp, _vp],
     "la_prec_info": [_vp, C.POINTER(_i), C.POINTER(_i)],
+    "la_prec_set_sweeps": [_vp, _i],
     "la_dot": [_i64, _i, _vp, _vp, C.POINTER(_d), _vp],
     "la_axpy_dot": [_i64, _i, _d, _vp, _vp, _vp, C.POINTER(_d), _vp],
     "la_solver_opts_default": [C.POINTER(SolverOpts)],

@@ -126,6 +126,167 @@ __device__ __forceinline__ double block_sum_partials(const double *__restrict__ 
   return t;
 }
 
+// ---- 16-byte vector forms of the CG kernels: each thread keeps four
+// 16-byte loads per operand in flight (the scalar grid-stride forms leave
+// these 2-16 MB streams latency-bound); used when every operand is 16-byte
+// aligned. Element i's thread and summation order are fixed by n and the
+// grid, so results stay run-to-run identical.
+template <typename V> struct V16;
+template <> struct V16<float> {
+  using T = float4;
+  static constexpr int k = 4;
+};
+template <> struct V16<double> {
+  using T = double2;
+  static constexpr int k = 2;
+};
+template <typename T, typename V>
+__device__ __forceinline__ V &vel(T &a, int e) {
+  return reinterpret_cast<V *>(&a)[e];
+}
+constexpr int kUn = 4;  // vectors in flight per operand and thread
+inline bool al16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+template <typename V>
+__global__ __launch_bounds__(kB) void k_dot_partial_v(int64_t n, const V *__restrict__ x,
+                                                      const V *__restrict__ y, double *part) {
+  using T = typename V16<V>::T;
+  constexpr int K = V16<V>::k;
+  __shared__ double sh[kB / 64];
+  const T *xv = reinterpret_cast<const T *>(x), *yv = reinterpret_cast<const T *>(y);
+  const int64_t nv = n / K, stride = static_cast<int64_t>(gridDim.x) * kB;
+  int64_t v = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  double acc = 0;
+  for (; v + (kUn - 1) * stride < nv; v += kUn * stride) {
+    T a[kUn], b[kUn];
+#pragma unroll
+    for (int u = 0; u < kUn; ++u) {
+      a[u] = xv[v + u * stride];
+      b[u] = yv[v + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < kUn; ++u)
+#pragma unroll
+      for (int e = 0; e < K; ++e)
+        acc += static_cast<double>(vel<T, V>(a[u], e)) * static_cast<double>(vel<T, V>(b[u], e));
+  }
+  for (; v < nv; v += stride) {
+    T a = xv[v], b = yv[v];
+#pragma unroll
+    for (int e = 0; e < K; ++e)
+      acc += static_cast<double>(vel<T, V>(a, e)) * static_cast<double>(vel<T, V>(b, e));
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * K) {
+    const int64_t i = nv * K + threadIdx.x;
+    acc += static_cast<double>(x[i]) * static_cast<double>(y[i]);
+  }
+  const double t = block_reduce<V>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+template <typename V>
+__global__ __launch_bounds__(kB) void k_cg_update_xr_v(int64_t n, V *x, V *r,
+                                                       const V *__restrict__ p,
+                                                       const V *__restrict__ q,
+                                                       const double *__restrict__ part_rz,
+                                                       const double *__restrict__ part_pq,
+                                                       double *part_rr) {
+  using T = typename V16<V>::T;
+  constexpr int K = V16<V>::k;
+  __shared__ double sh[kB / 64];
+  const double rz = block_sum_partials(part_rz, sh);
+  const double pq = block_sum_partials(part_pq, sh);
+  const V a = static_cast<V>(rz / pq);
+  T *xv = reinterpret_cast<T *>(x), *rv = reinterpret_cast<T *>(r);
+  const T *pv = reinterpret_cast<const T *>(p), *qv = reinterpret_cast<const T *>(q);
+  const int64_t nv = n / K, stride = static_cast<int64_t>(gridDim.x) * kB;
+  int64_t v = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  double acc = 0;
+  for (; v + (kUn - 1) * stride < nv; v += kUn * stride) {
+    T xa[kUn], ra[kUn], pa[kUn], qa[kUn];
+#pragma unroll
+    for (int u = 0; u < kUn; ++u) {
+      xa[u] = xv[v + u * stride];
+      ra[u] = rv[v + u * stride];
+      pa[u] = pv[v + u * stride];
+      qa[u] = qv[v + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < kUn; ++u) {
+#pragma unroll
+      for (int e = 0; e < K; ++e) {
+        vel<T, V>(xa[u], e) += a * vel<T, V>(pa[u], e);
+        const V rn = vel<T, V>(ra[u], e) - a * vel<T, V>(qa[u], e);
+        vel<T, V>(ra[u], e) = rn;
+        acc += static_cast<double>(rn) * static_cast<double>(rn);
+      }
+      xv[v + u * stride] = xa[u];
+      rv[v + u * stride] = ra[u];
+    }
+  }
+  for (; v < nv; v += stride) {
+    T xa = xv[v], ra = rv[v], pa = pv[v], qa = qv[v];
+#pragma unroll
+    for (int e = 0; e < K; ++e) {
+      vel<T, V>(xa, e) += a * vel<T, V>(pa, e);
+      const V rn = vel<T, V>(ra, e) - a * vel<T, V>(qa, e);
+      vel<T, V>(ra, e) = rn;
+      acc += static_cast<double>(rn) * static_cast<double>(rn);
+    }
+    xv[v] = xa;
+    rv[v] = ra;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * K) {
+    const int64_t i = nv * K + threadIdx.x;
+    x[i] += a * p[i];
+    const V rn = r[i] - a * q[i];
+    r[i] = rn;
+    acc += static_cast<double>(rn) * static_cast<double>(rn);
+  }
+  const double s = block_reduce<V>(acc, sh);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = s;
+}
+
+template <typename V>
+__global__ __launch_bounds__(kB) void k_cg_update_p_v(int64_t n, V *p, const V *__restrict__ z,
+                                                      const double *__restrict__ part_new,
+                                                      const double *__restrict__ part_old) {
+  using T = typename V16<V>::T;
+  constexpr int K = V16<V>::k;
+  __shared__ double sh[kB / 64];
+  const double rn = block_sum_partials(part_new, sh);
+  const double ro = block_sum_partials(part_old, sh);
+  const V b = static_cast<V>(rn / ro);
+  T *pv = reinterpret_cast<T *>(p);
+  const T *zv = reinterpret_cast<const T *>(z);
+  const int64_t nv = n / K, stride = static_cast<int64_t>(gridDim.x) * kB;
+  int64_t v = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  for (; v + (kUn - 1) * stride < nv; v += kUn * stride) {
+    T pa[kUn], za[kUn];
+#pragma unroll
+    for (int u = 0; u < kUn; ++u) {
+      pa[u] = pv[v + u * stride];
+      za[u] = zv[v + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < kUn; ++u) {
+#pragma unroll
+      for (int e = 0; e < K; ++e) vel<T, V>(pa[u], e) = vel<T, V>(za[u], e) + b * vel<T, V>(pa[u], e);
+      pv[v + u * stride] = pa[u];
+    }
+  }
+  for (; v < nv; v += stride) {
+    T pa = pv[v], za = zv[v];
+#pragma unroll
+    for (int e = 0; e < K; ++e) vel<T, V>(pa, e) = vel<T, V>(za, e) + b * vel<T, V>(pa, e);
+    pv[v] = pa;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * K) {
+    const int64_t i = nv * K + threadIdx.x;
+    p[i] = z[i] + b * p[i];
+  }
+}
+
 // x += alpha p ; r -= alpha q with alpha = <r,z> / <p,q> from their
 // partials; writes the partials of <r,r>
 template <typename V>
@@ -385,6 +546,17 @@ void fill_zero(int64_t n, la_dtype vec, void *x, hipStream_t s) {
 
 void dot_partials(int64_t n, la_dtype vec, const void *x, const void *y, double *partials,
                   hipStream_t s) {
+  if (al16(x) && al16(y)) {
+    if (vec == LA_F64)
+      hipLaunchKernelGGL(k_dot_partial_v<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                         static_cast<const double *>(x), static_cast<const double *>(y),
+                         partials);
+    else
+      hipLaunchKernelGGL(k_dot_partial_v<float>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                         static_cast<const float *>(x), static_cast<const float *>(y), partials);
+    LA_HIP(hipGetLastError());
+    return;
+  }
   if (vec == LA_F64)
     hipLaunchKernelGGL(k_dot_partial<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
                        static_cast<const double *>(x), static_cast<const double *>(y), partials);
@@ -402,6 +574,20 @@ double sum_partials_host(const double *h) {
 
 void cg_update_xr(int64_t n, la_dtype vec, void *x, void *r, const void *p, const void *q,
                   const double *part_rz, const double *part_pq, double *part_rr, hipStream_t s) {
+  if (al16(x) && al16(r) && al16(p) && al16(q)) {
+    if (vec == LA_F64)
+      hipLaunchKernelGGL(k_cg_update_xr_v<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                         static_cast<double *>(x), static_cast<double *>(r),
+                         static_cast<const double *>(p), static_cast<const double *>(q),
+                         part_rz, part_pq, part_rr);
+    else
+      hipLaunchKernelGGL(k_cg_update_xr_v<float>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                         static_cast<float *>(x), static_cast<float *>(r),
+                         static_cast<const float *>(p), static_cast<const float *>(q), part_rz,
+                         part_pq, part_rr);
+    LA_HIP(hipGetLastError());
+    return;
+  }
   if (vec == LA_F64)
     hipLaunchKernelGGL(k_cg_update_xr<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
                        static_cast<double *>(x), static_cast<double *>(r),
@@ -417,6 +603,18 @@ void cg_update_xr(int64_t n, la_dtype vec, void *x, void *r, const void *p, cons
 
 void cg_update_p(int64_t n, la_dtype vec, void *p, const void *z, const double *part_new,
                  const double *part_old, hipStream_t s) {
+  if (al16(p) && al16(z)) {
+    if (vec == LA_F64)
+      hipLaunchKernelGGL(k_cg_update_p_v<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                         static_cast<double *>(p), static_cast<const double *>(z), part_new,
+                         part_old);
+    else
+      hipLaunchKernelGGL(k_cg_update_p_v<float>, dim3(kRedBlocks), dim3(kB), 0, s, n,
+                         static_cast<float *>(p), static_cast<const float *>(z), part_new,
+                         part_old);
+    LA_HIP(hipGetLastError());
+    return;
+  }
   if (vec == LA_F64)
     hipLaunchKernelGGL(k_cg_update_p<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
                        static_cast<double *>(p), static_cast<const double *>(z), part_new, part_old);

@@ -400,6 +400,12 @@ int la_trsv_info(la_trsv_t T, int *n, int *nlevels, int64_t *nnz) {
   if (nnz) *nnz = T->t->nnz;
   LA_CATCH
 }
+int la_trsv_set_sweeps(la_trsv_t T, int sweeps) {
+  LA_TRY
+  LA_CHECK_ARG(T, "null handle");
+  trsv_set_sweeps(*T->t, sweeps);
+  LA_CATCH
+}
 int la_trsv_solve(la_trsv_t T, la_trsv_mode mode, la_dtype val, la_dtype vec, const void *b,
                   void *x) {
   LA_TRY
@@ -521,6 +527,13 @@ int la_prec_info(la_prec_t P, int *levels_lower, int *levels_upper) {
   if (levels_upper) *levels_upper = P->p->upper->nlevels;
   LA_CATCH
 }
+int la_prec_set_sweeps(la_prec_t P, int sweeps) {
+  LA_TRY
+  LA_CHECK_ARG(P, "null handle");
+  trsv_set_sweeps(*P->p->lower, sweeps);
+  trsv_set_sweeps(*P->p->upper, sweeps);
+  LA_CATCH
+}
 
 // ------------------------------------------------------------ BLAS-1
 namespace {
@@ -572,6 +585,7 @@ void la_solver_opts_default(la_solver_opts *o) {
   o->inner_tol = 1e-5;
   o->max_outer = 20;
   o->print_level = 0;
+  o->reliable_delta = 0.1;
 }
 
 int la_pcg_solve(la_mat_t A, la_prec_t P, const la_solver_opts *opts, const double *b,

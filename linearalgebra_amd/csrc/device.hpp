@@ -41,6 +41,11 @@ std::unique_ptr<Mat> make_mat(int rows, int cols, int base, const int *ai, const
 void spmv_prepare_segments(Mat &A);
 // builds the sliced-ELL copy; false when the matrix is not suited to it
 bool spmv_prepare_sell(Mat &A);
+// y = rs o (z - A x) with the sliced-ELL kernel (rs: row scale in the value
+// precision); false, with nothing launched, when the matrix has no
+// sliced-ELL form. FP64 accumulation for FP64 values and vectors.
+bool spmv_sell_rowscaled(Mat &A, la_dtype val, la_dtype vec, const void *rs, const void *x,
+                         const void *z, void *y, hipStream_t s);
 // the kernel family a request runs on (SEGSUM shares ALBUS's kernels)
 la_spmv_alg spmv_resolve(Mat &M, la_spmv_alg alg);
 // rowwiseSqrtNorm / DtAD on the device matrix (d: device, rows entries)
@@ -88,6 +93,13 @@ struct TrsvPlan {
   const double *fsrc = nullptr;
   // captured per-level launch sequences, keyed by (val, vec, b, x)
   std::map<std::tuple<int, int, const void *, void *>, hipGraphExec_t> graphs;
+  // JACOBI mode (built on first use): x = D^-1 w_k with w_0 = b and
+  // w_{s+1} = b - (N D^-1) w_s, N the strict triangle in row order
+  int sweeps = 3;
+  std::unique_ptr<Mat> jac;  // N D^-1
+  MultiValues jdinv;         // D^-1 (non-unit triangles)
+  DevBuf jtmp[2];            // sweep vectors
+  DevBuf jsrc, jdsrc;        // device-factor sources, row order
   ~TrsvPlan();
 };
 
@@ -105,6 +117,8 @@ void trsv_set_sources(TrsvPlan &T, const int64_t *src, const int64_t *dsrc, hipS
 // gathers the plan's values (and 1/diagonal) from the factor array f, then
 // refreshes the FP32/BF16 copies and any transposed plan
 void trsv_refresh(TrsvPlan &T, const double *f, hipStream_t s);
+// JACOBI mode sweep count for T and its transposed plan (0..1000000)
+void trsv_set_sweeps(TrsvPlan &T, int sweeps);
 // checks (and clears) the sync-free spin-bound error flag; synchronizes s
 bool trsv_check_timeout(TrsvPlan &T, hipStream_t s);
 

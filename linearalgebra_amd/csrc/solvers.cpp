@@ -8,10 +8,20 @@
 //    pass (one multi-dot + one multi-axpy per pass), Givens rotations on
 //    the host; stops on the Arnoldi residual estimate / ||b|| (:335-346) or
 //    a Hessenberg breakdown h_{k+1,k} < 1e-16 (:353-367).
-// Mixed precision: the inner solve runs with FP32 vectors and FP32/BF16
-// matrix and factors (FP32 accumulation, FP64 dot products); an outer FP64
-// iterative refinement computes r = b - A x with the FP64 matrix, solves
-// A d = r / ||r|| approximately, and updates x += ||r|| d in FP64.
+// Mixed precision: the iteration runs with FP32 vectors and FP32/BF16
+// matrix and factors (FP32 accumulation, FP64 dot products); x and the
+// true residual r = b - A x (FP64 matrix) stay in FP64.
+//  * PCG, reliable_delta > 0: reliable updates (Sleijpen & van der Vorst,
+//    "Reliable updated residuals in hybrid Bi-CG methods", Computing 56,
+//    1996; Clark et al., "Solving lattice QCD systems of equations using
+//    mixed precision solvers on GPUs", 2010): one PCG iteration on b/||b||
+//    whose FP32 increment of x is folded into the FP64 x, and whose
+//    residual is replaced by the FP64 true residual, each time the
+//    recurrence residual falls below delta times its largest value since
+//    the last replacement, and before stopping. The search direction is
+//    kept across a replacement, so the Krylov space is not thrown away.
+//  * otherwise (and FGMRES): restarted iterative refinement -- solve
+//    A d = r / ||r|| approximately, x += ||r|| d in FP64, repeat.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -29,6 +39,10 @@ struct Work {
   DevBuf V, Z;  // GMRES bases
   la_dtype basis_dt = LA_F64;
   Scratch sc;
+  hipEvent_t ev = nullptr;  // marks the residual-norm copy of an iteration
+  ~Work() {
+    if (ev) (void)hipEventDestroy(ev);
+  }
 };
 
 thread_local std::unique_ptr<Work> tl_work;
@@ -41,6 +55,7 @@ Work &work_for(int64_t n) {
       tl_work->v64[i].alloc(sizeof(double) * std::max<int64_t>(n, 1));
       tl_work->v32[i].alloc(sizeof(float) * std::max<int64_t>(n, 1));
     }
+    LA_HIP(hipEventCreateWithFlags(&tl_work->ev, hipEventDisableTiming));
   }
   return *tl_work;
 }
@@ -134,7 +149,14 @@ CgResult pcg_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_dty
       p_check = p_tt;
     }
     LA_HIP(hipMemcpyAsync(W.sc.host, p_check, sizeof(double) * kRedBlocks, hipMemcpyDeviceToHost, s));
-    LA_HIP(hipStreamSynchronize(s));
+    LA_HIP(hipEventRecord(W.ev, s));
+    // the next direction is queued before the stop test is read, so the
+    // device runs it while the host waits; a converged run discards it
+    // (x and r are final before it)
+    apply_M(P, o, val, vec, r, z, n, s);
+    dot_partials(n, vec, r, z, p_rz[cur ^ 1], s);
+    cg_update_p(n, vec, p, z, p_rz[cur ^ 1], p_rz[cur], s);
+    LA_HIP(hipEventSynchronize(W.ev));
     const double res = std::sqrt(sum_partials_host(W.sc.host));
     R.iters = it;
     R.last_res = res;
@@ -143,11 +165,9 @@ CgResult pcg_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_dty
       R.converged = std::isfinite(res);
       break;
     }
-    apply_M(P, o, val, vec, r, z, n, s);
-    dot_partials(n, vec, r, z, p_rz[cur ^ 1], s);
-    cg_update_p(n, vec, p, z, p_rz[cur ^ 1], p_rz[cur], s);
     cur ^= 1;
   }
+  LA_HIP(hipStreamSynchronize(s));
   return R;
 }
 
@@ -243,13 +263,91 @@ CgResult fgmres_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_
   return R;
 }
 
+// Mixed-precision PCG with reliable updates (see the file comment). x: FP64
+// device vector holding x0; vd: the inner matrix precision. Counts the true
+// residual evaluations in *updates.
+CgResult pcg_reliable(Mat &A, Prec *P, const la_solver_opts &o, la_dtype vd, const double *b,
+                      double *x, double nb, Work &W, hipStream_t s, int *updates) {
+  const int64_t n = A.rows;
+  const la_spmv_alg alg = static_cast<la_spmv_alg>(o.spmv_alg);
+  void *r = W.v32[0].p, *z = W.v32[1].p, *p = W.v32[2].p, *q = W.v32[3].p, *xl = W.v32[4].p;
+  double *r64 = W.v64[5].as<double>();
+  double *part = W.sc.partials.as<double>();
+  double *p_pq = part, *p_rr = part + kRedBlocks;
+  double *p_rz[2] = {part + 3 * kRedBlocks, part + 4 * kRedBlocks};
+  const double goal = o.abs_tol > 0 ? std::max(o.rel_tol * nb, o.abs_tol) : o.rel_tol * nb;
+  const double delta = o.reliable_delta;
+  // x += nb * xl; xl = 0; r64 = b - A x; r = r64 / nb (FP32); returns ||r64||
+  bool first = true;
+  auto replace = [&]() {
+    if (!first) add_scaled(n, LA_F32, xl, nb, x, s);
+    first = false;
+    fill_zero(n, LA_F32, xl, s);
+    spmv(A, alg, LA_F64, LA_F64, LA_F64, -1.0, x, 1.0, b, r64, s);
+    const double rn = norm2(n, LA_F64, r64, W.sc, s);
+    copy_vec(n, LA_F64, r64, LA_F32, r, 1.0 / nb, s);
+    ++*updates;
+    if (o.print_level == 1) std::fprintf(stderr, "true residual: %lg/%lg=%lg\n", rn, nb, rn / nb);
+    return rn;
+  };
+  CgResult R;
+  double rn = replace();
+  R.last_res = rn;
+  if (rn < goal) {
+    R.converged = true;
+    return R;
+  }
+  int cur = 0;
+  apply_M(P, o, vd, LA_F32, r, z, n, s);
+  copy_vec(n, LA_F32, z, LA_F32, p, 1.0, s);
+  dot_partials(n, LA_F32, r, z, p_rz[cur], s);
+  double rmax = rn / nb;  // in units of ||b||, like r
+  bool folded = true;     // xl holds nothing x lacks
+  for (int it = 1; it <= o.max_iters; ++it) {
+    spmv(A, alg, vd, LA_F32, LA_F32, 1.0, p, 0.0, q, q, s);
+    dot_partials(n, LA_F32, p, q, p_pq, s);
+    cg_update_xr(n, LA_F32, xl, r, p, q, p_rz[cur], p_pq, p_rr, s);
+    folded = false;
+    LA_HIP(hipMemcpyAsync(W.sc.host, p_rr, sizeof(double) * kRedBlocks, hipMemcpyDeviceToHost, s));
+    LA_HIP(hipEventRecord(W.ev, s));
+    // next direction queued before the test is read (see pcg_core); after a
+    // replacement it stays the one built from the recurrence residual, a
+    // valid search direction, while the residual itself is the true one
+    apply_M(P, o, vd, LA_F32, r, z, n, s);
+    dot_partials(n, LA_F32, r, z, p_rz[cur ^ 1], s);
+    cg_update_p(n, LA_F32, p, z, p_rz[cur ^ 1], p_rz[cur], s);
+    LA_HIP(hipEventSynchronize(W.ev));
+    const double rr = std::sqrt(sum_partials_host(W.sc.host));
+    R.iters = it;
+    if (o.print_level == 1) std::fprintf(stderr, "%3d = %lg (recurrence, relative)\n", it, rr);
+    if (!std::isfinite(rr)) break;
+    rmax = std::max(rmax, rr);
+    if (rr < delta * rmax || rr * nb < goal) {
+      rn = replace();
+      folded = true;
+      R.last_res = rn;
+      if (rn < goal) {
+        R.converged = true;
+        break;
+      }
+      if (!std::isfinite(rn)) break;
+      rmax = rn / nb;
+    }
+    cur ^= 1;
+  }
+  if (!folded) add_scaled(n, LA_F32, xl, nb, x, s);  // x keeps the last iterate
+  LA_HIP(hipStreamSynchronize(s));
+  return R;
+}
+
 using clk = std::chrono::steady_clock;
 
 template <typename Core>
 int run_solver(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
-               la_solver_result *res, Core core) {
+               la_solver_result *res, Core core, bool is_pcg) {
   LA_CHECK_ARG(o.precision >= LA_PREC_FP64 && o.precision <= LA_PREC_MIXED_BF16_F32P,
                "unknown precision policy");
+  LA_CHECK_ARG(!(o.reliable_delta >= 1.0), "reliable_delta must be below 1");
   hipStream_t s = A.stream;
   const int64_t n = A.rows;
   Work &W = work_for(n);
@@ -266,6 +364,15 @@ int run_solver(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double
                       o.rel_tol, o.abs_tol, o.max_iters, W);
     out.iters = R.iters;
     out.outer_iters = 1;
+    out.converged = R.converged;
+  } else if (is_pcg && o.reliable_delta > 0) {
+    const la_dtype vd = inner_val_dtype(o.precision);
+    if (!A.vals.has[vd]) A.vals.add(vd, s);
+    if (P) prec_add_dtype(*P, inner_prec_dtype(o.precision));
+    int updates = 0;
+    CgResult R = pcg_reliable(A, P, o, vd, b, x, nb, W, s, &updates);
+    out.iters = R.iters;
+    out.outer_iters = updates;
     out.converged = R.converged;
   } else {
     const la_dtype vd = inner_val_dtype(o.precision);
@@ -338,7 +445,8 @@ int pcg_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double 
                         double rt, double at, int mi, Work &W) {
                       const bool true_res = vec == LA_F64 && o.stop_rule == LA_STOP_TRUE_RESIDUAL;
                       return pcg_core(A, P, o, val, vec, bb, xx, nb, rt, at, mi, true_res, W, s);
-                    });
+                    },
+                    true);
 }
 
 int fgmres_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
@@ -351,7 +459,8 @@ int fgmres_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, doub
                     [&](la_dtype val, la_dtype vec, const void *bb, void *xx, double nb,
                         double rt, double at, int mi, Work &W) {
                       return fgmres_core(A, P, o, val, vec, bb, xx, nb, rt, at, mi, m, W, s);
-                    });
+                    },
+                    false);
 }
 
 // FGMRES on A R^-1 u = b (mkl_fgmres_solver with a right operator R,

@@ -327,7 +327,7 @@ template <typename T, typename V, typename A>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     int n, int64_t nslices, int uw, const int64_t *__restrict__ sp,
     const int *__restrict__ sw, const int *__restrict__ scol, const T *__restrict__ sval,
-    const V *__restrict__ x, V *y, const V *z, A alpha, A beta) {
+    const V *__restrict__ x, V *y, const V *z, A alpha, A beta, const T *__restrict__ rs) {
   const int64_t sl = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * (kBlock / 64) +
                      (threadIdx.x >> 6);
   if (sl >= nslices) return;
@@ -336,6 +336,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
   // uw > 0: every slice has that width (no slice table to read)
   const int64_t base = (uw > 0 ? sl * 64 * uw : sp[sl]) + lane;
   const int w = uw > 0 ? uw : sw[sl];
+  // the epilogue's operands are loaded first, so their latency overlaps the
+  // row's loads instead of following them (z may alias y: each lane reads
+  // its own element before writing it)
+  A zr = 0, sr = 1;
+  if (row < n) {
+    if (beta != A(0)) zr = A(z[row]);
+    if (rs) sr = static_cast<A>(ldm(rs, row));
+  }
   A acc = 0;
 #define LA_SB(WW, NT) acc = sell_block<WW, NT, T, V, A>(acc, k, base, scol, sval, x)
   int k = 0;
@@ -371,7 +379,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
 #undef LA_SB
   if (row < n) {
     A o = alpha * acc;
-    if (beta != A(0)) o += beta * A(z[row]);
+    if (beta != A(0)) o += beta * zr;
+    if (rs) o *= sr;  // y = rs o (alpha A x + beta z)
     y[row] = V(o);
   }
 }
@@ -456,13 +465,14 @@ template <> struct DtOf<bf16> { static constexpr la_dtype v = LA_BF16; };
 
 template <typename T, typename V, typename A>
 void launch_sell(Mat &M, const T *av, const V *x, V *y, const V *z, A alpha, A beta,
-                 hipStream_t s) {
+                 hipStream_t s, const T *rs = nullptr) {
   const T *sv = sell_values<T>(M, DtOf<T>::v, av, s);
-  const unsigned grid = static_cast<unsigned>((M.sell_nslices + kBlock / 64 - 1) / (kBlock / 64));
+  constexpr int wpb = kBlock / 64;  // wavefronts per workgroup
+  const unsigned grid = static_cast<unsigned>((M.sell_nslices + wpb - 1) / wpb);
   hipLaunchKernelGGL((k_spmv_sell<T, V, A>), dim3(grid), dim3(kBlock), 0, s, M.rows,
                      M.sell_nslices, M.sell_uniform_w, M.sell_ptr.as<int64_t>(),
                      M.sell_w.as<int>(),
-                     M.sell_col.as<int>(), sv, x, y, z, alpha, beta);
+                     M.sell_col.as<int>(), sv, x, y, z, alpha, beta, rs);
   LA_HIP(hipGetLastError());
 }
 
@@ -527,6 +537,30 @@ std::unique_ptr<Mat> make_mat(int rows, int cols, int base, const int *ai, const
 }
 
 bool spmv_prepare_sell(Mat &A) { return prepare_sell(A); }
+
+bool spmv_sell_rowscaled(Mat &A, la_dtype val, la_dtype vec, const void *rs, const void *x,
+                         const void *z, void *y, hipStream_t s) {
+  LA_CHECK_ARG(val >= LA_F64 && val <= LA_BF16 && (vec == LA_F64 || vec == LA_F32),
+               "bad dtype");
+  if (A.rows == 0 || A.nnz == 0 || A.cols == 0 || !prepare_sell(A)) return false;
+  if (!A.vals.has[val]) A.vals.add(val, s);
+  const void *av = A.vals.get(val);
+#define LA_RS(T, V, AC)                                                                        \
+  launch_sell<T, V, AC>(A, static_cast<const T *>(av), static_cast<const V *>(x),              \
+                        static_cast<V *>(y), static_cast<const V *>(z), AC(-1), AC(1), s,      \
+                        static_cast<const T *>(rs))
+  if (vec == LA_F64) {
+    if (val == LA_F64) LA_RS(double, double, double);
+    else if (val == LA_F32) LA_RS(float, double, float);
+    else LA_RS(bf16, double, float);
+  } else {
+    if (val == LA_F64) LA_RS(double, float, float);
+    else if (val == LA_F32) LA_RS(float, float, float);
+    else LA_RS(bf16, float, float);
+  }
+#undef LA_RS
+  return true;
+}
 
 la_spmv_alg spmv_resolve(Mat &M, la_spmv_alg alg) {
   if (M.nnz == 0 || M.cols == 0) return LA_SPMV_ROW;

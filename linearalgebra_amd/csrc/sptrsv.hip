@@ -18,6 +18,16 @@
 //    ticket, which a running workgroup already holds: the solve always
 //    drains. Waits are bounded; a timeout sets an error flag instead of
 //    hanging.
+//  * JACOBI (no counterpart in the reference): k Jacobi sweeps on the
+//    triangle, y_0 = D^-1 b, y_{s+1} = D^-1 (b - N y_s), each sweep an
+//    SpMV with the column-scaled strict triangle N D^-1. D^-1 N is nilpotent (its
+//    index is the number of levels), so k >= levels - 1 sweeps give the
+//    exact solution; fewer give the truncated Neumann series, an
+//    approximate solve whose cost is k bandwidth-bound passes instead of a
+//    chain of level-to-level dependencies (Anzt, Chow, Dongarra, "Iterative
+//    sparse triangular solves for preconditioning", Euro-Par 2015). For an
+//    IC factor both triangles use the same k, so the applied operator
+//    P_k^T P_k stays symmetric positive definite and PCG stays valid.
 // Row formula (triangle_solve.hpp:32-71): x_i = (b_i - sum_j T_ij x_j) / d_i,
 // computed as a product with 1/d_i stored in the plan (within an ulp of the
 // quotient; keeps a divide off the sync-free dependency chain).
@@ -295,6 +305,135 @@ void solve_dispatch(TrsvPlan &P, la_trsv_mode mode, la_dtype val, la_dtype vec, 
   return solve_typed<bf16, float, float>(P, mode, v, d, b, x, s);
 }
 
+// ------------------------------------------------------------ JACOBI mode
+// Sweeps in the column-scaled form: with w = D y, y_{s+1} = D^-1 (b - N y_s)
+// becomes w_{s+1} = b - (N D^-1) w_s from w_0 = b, and x = D^-1 w_k; the
+// last sweep applies D^-1 in its epilogue, so a solve is k launches (no
+// separate scaling pass).
+
+// out_i = dinv_i * b_i (k = 0, or the last sweep without a sliced-ELL form)
+template <typename T, typename V, typename A>
+__global__ void k_jac_scale(int n, const T *__restrict__ dinv, const V *__restrict__ b, V *out) {
+  const int i = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = static_cast<V>(static_cast<A>(ldt(dinv, i)) * static_cast<A>(b[i]));
+}
+
+// D^-1 from a device factor array f
+__global__ void k_jac_dinv(int n, const int64_t *__restrict__ dsrc, const double *__restrict__ f,
+                           double *dinv) {
+  const int i = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dinv[i] = 1.0 / f[dsrc[i]];
+}
+
+// N D^-1 from f, row by row (dinv null: unit diagonal)
+__global__ void k_jac_vals(int n, const int *__restrict__ ai, const int *__restrict__ aj,
+                           const int64_t *__restrict__ src, const double *__restrict__ f,
+                           const double *__restrict__ dinv, double *nv) {
+  const int i = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int q = ai[i]; q < ai[i + 1]; ++q) nv[q] = dinv ? f[src[q]] * dinv[aj[q]] : f[src[q]];
+}
+
+void jacobi_refresh(TrsvPlan &T, const double *f, hipStream_t s) {
+  Mat &J = *T.jac;
+  if (T.h_src.empty() || T.n == 0) return;
+  const unsigned grid = static_cast<unsigned>((T.n + 255) / 256);
+  double *dv = T.unit ? nullptr : T.jdinv.v[LA_F64].as<double>();
+  if (!T.unit && !T.h_dsrc.empty()) {
+    hipLaunchKernelGGL(k_jac_dinv, dim3(grid), dim3(256), 0, s, T.n, T.jdsrc.as<int64_t>(), f,
+                       dv);
+    LA_HIP(hipGetLastError());
+    T.jdinv.set_from_device_f64(dv, T.n, s);
+  }
+  if (J.nnz) {
+    double *nv = J.vals.v[LA_F64].as<double>();
+    hipLaunchKernelGGL(k_jac_vals, dim3(grid), dim3(256), 0, s, T.n, J.ai.as<int>(),
+                       J.aj.as<int>(), T.jsrc.as<int64_t>(), f, dv, nv);
+    LA_HIP(hipGetLastError());
+    J.vals.set_from_device_f64(nv, J.nnz, s);
+  }
+}
+
+void build_jacobi(TrsvPlan &T, hipStream_t s) {
+  const int n = T.n;
+  std::vector<double> sv(static_cast<size_t>(T.nnz)), dv(T.unit ? 0 : n);
+  for (int i = 0; i < n && !T.unit; ++i) dv[i] = 1.0 / T.h_diag[i];
+  for (int i = 0; i < n; ++i)
+    for (int q = T.h_ai[i]; q < T.h_ai[i + 1]; ++q)
+      sv[q] = T.unit ? T.h_av[q] : T.h_av[q] * dv[T.h_aj[q]];
+  T.jac = make_mat(n, n, 0, T.h_ai.data(), T.h_aj.data(), sv.data(), s);
+  if (!T.unit) T.jdinv.set_from_host(dv.data(), n, s);
+  T.jtmp[0].alloc(sizeof(double) * static_cast<size_t>(n));
+  T.jtmp[1].alloc(sizeof(double) * static_cast<size_t>(n));
+  if (!T.h_src.empty()) {
+    T.jsrc.alloc(sizeof(int64_t) * std::max<size_t>(T.h_src.size(), 1));
+    T.jsrc.upload(T.h_src.data(), sizeof(int64_t) * T.h_src.size(), s);
+    if (!T.h_dsrc.empty()) {
+      T.jdsrc.alloc(sizeof(int64_t) * T.h_dsrc.size());
+      T.jdsrc.upload(T.h_dsrc.data(), sizeof(int64_t) * T.h_dsrc.size(), s);
+    }
+  }
+  if (T.fsrc) jacobi_refresh(T, T.fsrc, s);
+  LA_HIP(hipStreamSynchronize(s));
+}
+
+void jac_scale(int n, la_dtype val, la_dtype vec, const void *d, const void *b, void *o,
+               hipStream_t s) {
+  const dim3 grid(static_cast<unsigned>((n + 255) / 256)), blk(256);
+#define LA_JS(T, V, A)                                                                          \
+  hipLaunchKernelGGL((k_jac_scale<T, V, A>), grid, blk, 0, s, n, static_cast<const T *>(d),     \
+                     static_cast<const V *>(b), static_cast<V *>(o))
+  if (vec == LA_F64) {
+    if (val == LA_F64) LA_JS(double, double, double);
+    else if (val == LA_F32) LA_JS(float, double, float);
+    else LA_JS(bf16, double, float);
+  } else {
+    if (val == LA_F64) LA_JS(double, float, float);
+    else if (val == LA_F32) LA_JS(float, float, float);
+    else LA_JS(bf16, float, float);
+  }
+#undef LA_JS
+  LA_HIP(hipGetLastError());
+}
+
+void jacobi_solve(TrsvPlan &T, la_dtype val, la_dtype vec, const void *b, void *x,
+                  hipStream_t s) {
+  if (!T.jac) build_jacobi(T, s);
+  Mat &J = *T.jac;
+  const int n = T.n;
+  const size_t esz = vec == LA_F64 ? 8 : 4;
+  const la_dtype acc = val == LA_F64 && vec == LA_F64 ? LA_F64 : LA_F32;
+  const int k = T.sweeps;
+  const void *d = nullptr;
+  if (!T.unit) {
+    if (!T.jdinv.has[val]) T.jdinv.add(val, s);
+    d = T.jdinv.get(val);
+  }
+  if (k == 0) {
+    if (d) jac_scale(n, val, vec, d, b, x, s);
+    else LA_HIP(hipMemcpyAsync(x, b, esz * n, hipMemcpyDeviceToDevice, s));
+    return;
+  }
+  void *t0 = T.jtmp[0].p, *t1 = T.jtmp[1].p;
+  // sweeps 1 .. k-1 alternate between the two spare vectors (a unit
+  // triangle's sweep q writes x when k - q is even instead)
+  const void *prev = b;
+  for (int q = 1; q < k; ++q) {
+    void *dst = d ? (q % 2 ? t0 : t1) : ((k - q) % 2 == 0 ? x : (prev == t0 ? t1 : t0));
+    spmv(J, LA_SPMV_AUTO, val, vec, acc, -1.0, prev, 1.0, b, dst, s);
+    prev = dst;
+  }
+  if (!d) {
+    spmv(J, LA_SPMV_AUTO, val, vec, acc, -1.0, prev, 1.0, b, x, s);
+    return;
+  }
+  // last sweep: x = D^-1 (b - (N D^-1) w_{k-1})
+  if (spmv_sell_rowscaled(J, val, vec, d, prev, b, x, s)) return;
+  void *tmp = prev == t0 ? t1 : t0;
+  spmv(J, LA_SPMV_AUTO, val, vec, acc, -1.0, prev, 1.0, b, tmp, s);
+  jac_scale(n, val, vec, d, tmp, x, s);
+}
+
 hipStream_t capture_stream() {
   static thread_local hipStream_t cs = nullptr;
   if (!cs) LA_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
@@ -415,12 +554,17 @@ void trsv_add_dtype(TrsvPlan &T, la_dtype dt) {
 
 void trsv_solve(TrsvPlan &T, la_trsv_mode mode, la_dtype val, la_dtype vec, const void *b,
                 void *x, hipStream_t s) {
-  LA_CHECK_ARG(mode == LA_TRSV_LEVEL || mode == LA_TRSV_SYNCFREE, "bad trsv mode");
+  LA_CHECK_ARG(mode == LA_TRSV_LEVEL || mode == LA_TRSV_SYNCFREE || mode == LA_TRSV_JACOBI,
+               "bad trsv mode");
   LA_CHECK_ARG(val >= LA_F64 && val <= LA_BF16, "bad value dtype");
   LA_CHECK_ARG(vec == LA_F64 || vec == LA_F32, "vectors must be FP64 or FP32");
   LA_CHECK_ARG(T.n == 0 || (b && x), "null vector");
   LA_CHECK_ARG(b != x || T.n == 0, "b and x must be distinct");
   if (T.n == 0) return;
+  if (mode == LA_TRSV_JACOBI) {
+    jacobi_solve(T, val, vec, b, x, s);
+    return;
+  }
   if ((T.nnz && !T.val.has[val]) || (!T.unit && !T.diag.has[val])) trsv_add_dtype(T, val);
   if (mode == LA_TRSV_SYNCFREE || T.nlevels <= 2) {
     solve_dispatch(T, mode, val, vec, b, x, s);
@@ -461,6 +605,7 @@ TrsvPlan &trsv_transposed(TrsvPlan &T) {
               tv.data());
     T.transposed = make_trsv(T.n, 0, ti.data(), tj.data(), tv.data(),
                              T.unit ? nullptr : T.h_diag.data(), !T.upper, T.stream);
+    T.transposed->sweeps = T.sweeps;
     if (!T.h_src.empty() || !T.h_dsrc.empty()) {
       // the same transpose carries the source positions (exact as doubles)
       std::vector<double> sv(T.h_src.begin(), T.h_src.end()), st(static_cast<size_t>(T.nnz));
@@ -493,6 +638,7 @@ __global__ void k_gather_recip(int64_t m, const int64_t *__restrict__ src,
 
 void trsv_set_sources(TrsvPlan &T, const int64_t *src, const int64_t *dsrc, hipStream_t s) {
   const int n = T.n;
+  T.jac.reset();  // the JACOBI copy is rebuilt with the new sources
   T.h_src.assign(src, src + T.nnz);
   if (dsrc) T.h_dsrc.assign(dsrc, dsrc + n);
   else T.h_dsrc.clear();
@@ -530,7 +676,14 @@ void trsv_refresh(TrsvPlan &T, const double *f, hipStream_t s) {
     LA_HIP(hipGetLastError());
     T.diag.set_from_device_f64(d, T.n, s);
   }
+  if (T.jac) jacobi_refresh(T, f, s);
   if (T.transposed) trsv_refresh(*T.transposed, f, s);
+}
+
+void trsv_set_sweeps(TrsvPlan &T, int sweeps) {
+  LA_CHECK_ARG(sweeps >= 0 && sweeps <= 1000000, "sweeps out of range");
+  T.sweeps = sweeps;
+  if (T.transposed) trsv_set_sweeps(*T.transposed, sweeps);
 }
 
 bool trsv_check_timeout(TrsvPlan &T, hipStream_t s) {

@@ -250,6 +250,40 @@ void orc_forward_sub_t(int n, int base, const int *ai, const int *aj,
       x[aj[j] - base] -= av[j] * x[i];
 }
 
+/* No counterpart in the reference: k Jacobi sweeps on (D + T) x = b with T
+ * the strict triangle (either orientation) and D = diag (NULL = unit),
+ * y_0 = D^-1 b, y_{s+1} = D^-1 (b - T y_s), x = y_k -- in the column-scaled
+ * form the device runs: w_0 = b, w_{s+1} = b - (T D^-1) w_s with the
+ * entries T_ij * (1/d_j) rounded first, x = D^-1 w_k, the last sweep scaled
+ * by 1/d_i after its sum. */
+void orc_trsv_jacobi(int n, int base, const int *ai, const int *aj,
+                     const double *av, const double *diag, const double *b,
+                     int k, double *x) {
+  double *dinv = (double *)malloc(sizeof(double) * (n ? n : 1));
+  double *w = (double *)malloc(sizeof(double) * (n ? n : 1));
+  double *y = (double *)malloc(sizeof(double) * (n ? n : 1));
+  for (int i = 0; i < n; i++) {
+    dinv[i] = diag ? 1.0 / diag[i] : 1.0;
+    w[i] = b[i];
+  }
+  for (int s = 0; s < k; s++) {
+    for (int i = 0; i < n; i++) {
+      double acc = 0;
+      for (int j = ai[i] - base; j < ai[i + 1] - base; j++) {
+        const int c = aj[j] - base;
+        const double t = diag ? av[j] * dinv[c] : av[j];
+        acc += t * w[c];
+      }
+      y[i] = b[i] - acc;
+    }
+    memcpy(w, y, sizeof(double) * n);
+  }
+  for (int i = 0; i < n; i++) x[i] = diag ? dinv[i] * w[i] : w[i];
+  free(dinv);
+  free(w);
+  free(y);
+}
+
 /* sparse_mat_op/matrix_utils.hpp:637-675 */
 int orc_topo_sort2(int n, int base, const int *ai, const int *aj, int upper,
                    int *iperm, int *prefix) {

@@ -52,6 +52,14 @@ void orc_forward_sub(int n, int base, const int *ai, const int *aj,
 void orc_backward_sub(int n, int base, const int *ai, const int *aj,
                       const double *av, const double *diag, const double *b,
                       double *x);
+/* No counterpart in the reference (the device's JACOBI triangular-solve
+ * mode): k Jacobi sweeps, y_0 = D^-1 b, y_{s+1} = D^-1 (b - T y_s), in the
+ * column-scaled form w = D y the device runs.
+ * pinned (k >= levels - 1 reproduces orc_forward_sub / orc_backward_sub:
+ * D^-1 T is nilpotent; k = 0 is D^-1 b; scipy dense Neumann series). */
+void orc_trsv_jacobi(int n, int base, const int *ai, const int *aj,
+                     const double *av, const double *diag, const double *b,
+                     int k, double *x);
 /* triangle_solve.hpp:87-98 ((I+U^T) x = b from CSR of strict U). pinned. */
 void orc_forward_sub_t(int n, int base, const int *ai, const int *aj,
                        const double *av, const double *b, double *x);

@@ -51,6 +51,7 @@ def _declare(L):
     L.orc_forward_sub.argtypes = [i, i, _I, _I, _D, _D, _D]
     L.orc_backward_sub.argtypes = [i, i, _I, _I, _D, _D, _D, _D]
     L.orc_forward_sub_t.argtypes = [i, i, _I, _I, _D, _D, _D]
+    L.orc_trsv_jacobi.argtypes = [i, i, _I, _I, _D, C.c_void_p, _D, i, _D]
     L.orc_topo_sort2.argtypes = [i, i, _I, _I, i, _I, _I]
     L.orc_topo_sort2.restype = i
     L.orc_level_forward.argtypes = [_I, _I, i, i, i, _I, _I, _D, _D, _D]
@@ -147,6 +148,15 @@ def backward_sub(U, diag, b):
 def forward_sub_t(U, b):
     x = np.zeros(U.n)
     lib().orc_forward_sub_t(U.n, U.base, U.ai, U.aj, U.av, _f64(b), x)
+    return x
+
+
+def trsv_jacobi(T, diag, b, k):
+    """k Jacobi sweeps on (D + T) x = b (diag None: unit)."""
+    x = np.zeros(T.n)
+    d = None if diag is None else _f64(diag)
+    lib().orc_trsv_jacobi(T.n, T.base, T.ai, T.aj, T.av,
+                          None if d is None else d.ctypes.data_as(C.c_void_p), _f64(b), k, x)
     return x
 
 

@@ -29,8 +29,18 @@ def _cases():
           ("all_empty", lambda: orc.CSR(7, np.zeros(8), [], [])),
           ("one_by_one", lambda: from_dense([[3.0]])),
           ("K_bin_base1", lambda: _bin("K.bin")),
-          ("G2_bin_rect", lambda: _bin("G2.bin"))]
+          ("G2_bin_rect", lambda: _bin("G2.bin")),
+          # narrow rows (at most 4 per row: two slices per wavefront), with
+          # an odd slice count and a ragged last slice
+          ("lower_laplace3d_12", lambda: orc.split_triangle(_o(la.gen.laplace3d(12)), False)),
+          ("bidiag_273", lambda: _bidiag(273))]
     return cs
+
+
+def _bidiag(n):
+    D = np.diag(np.full(n, 2.0)) + np.diag(np.full(n - 1, -1.0), -1)
+    D[5, :] = 0.0  # an empty row
+    return from_dense(D)
 
 
 def _o(A):
