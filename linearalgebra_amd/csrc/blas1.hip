@@ -2,6 +2,9 @@
 // (deterministic two-stage reduction), fused CG updates with device-side
 // scalars, multi-dot / multi-axpy for GMRES orthogonalisation, precision
 // conversions.
+#include <chrono>
+#include <cstring>
+
 #include "dev_util.hpp"
 #include "device.hpp"
 
@@ -469,9 +472,56 @@ Scratch::Scratch() {
   scalars.alloc(sizeof(double) * 256);
   LA_HIP(hipHostMalloc(reinterpret_cast<void **>(&host), sizeof(double) * 2 * kRedBlocks,
                        hipHostMallocDefault));
+  LA_HIP(hipHostMalloc(reinterpret_cast<void **>(&rep), sizeof(uint64_t) * 32,
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  rep[0] = rep[1] = 0;
+  LA_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&rep_dev), rep, 0));
 }
 Scratch::~Scratch() {
   if (host) (void)hipHostFree(host);
+  if (rep) (void)hipHostFree(rep);
+}
+
+namespace {
+// one workgroup: the fixed-order sum of the partials (block_sum_partials'
+// order), then the value and the sequence number into host memory with
+// system-scope stores, the number last (release)
+__global__ __launch_bounds__(kB) void k_report(const double *__restrict__ part, uint64_t *slot,
+                                               uint64_t seq) {
+  __shared__ double sh[kB / 64];
+  const double t = block_sum_partials(part, sh);
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(slot, __double_as_longlong(t), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(slot + 1, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+}  // namespace
+
+uint64_t report_sum(Scratch &S, const double *part, hipStream_t s) {
+  const uint64_t seq = ++S.rep_seq;
+  hipLaunchKernelGGL(k_report, dim3(1), dim3(kB), 0, s, part, S.rep_dev, seq);
+  LA_HIP(hipGetLastError());
+  return seq;
+}
+
+double wait_report(Scratch &S, uint64_t seq, hipStream_t s) {
+  // spin on the coherent slot; after ~2 s without it, let the runtime
+  // report a stream error (or the value, if it came meanwhile)
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint64_t spins = 0;; ++spins) {
+    if (__atomic_load_n(S.rep + 1, __ATOMIC_ACQUIRE) == seq) break;
+    if ((spins & 4095) == 4095 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+      LA_HIP(hipStreamSynchronize(s));
+      if (__atomic_load_n(S.rep + 1, __ATOMIC_ACQUIRE) == seq) break;
+      throw la_error(LA_ERR_HIP, "device report did not arrive");
+    }
+  }
+  const uint64_t bits = __atomic_load_n(S.rep, __ATOMIC_RELAXED);
+  double v;
+  std::memcpy(&v, &bits, sizeof v);
+  return v;
 }
 
 void dot_dev(int64_t n, la_dtype vec, const void *x, const void *y, double *partials, double *out,

@@ -28,6 +28,7 @@ struct Mat {
   int sell_state = 0;  // 0 not built, 1 built, -1 not worth it (too much padding)
   int64_t sell_nslices = 0, sell_len = 0;
   int sell_uniform_w = 0;  // > 0 when every slice has this width
+  bool sell_col16 = false; // columns stored as 16-bit offsets from the row
   DevBuf sell_ptr, sell_w, sell_col, sell_src, sell_val[3];
   uint64_t sell_ver[3] = {0, 0, 0};
   // A^T, built on first transposed product
@@ -166,9 +167,18 @@ struct Scratch {
   DevBuf partials;  // kRedBlocks * kMaxDots doubles
   DevBuf scalars;   // device scalars used by the solvers
   double *host = nullptr;  // pinned mirror of a few scalars
+  // coherent pinned report slot {value, sequence number}, written by the
+  // device (report_sum) and polled by the host (wait_report)
+  uint64_t *rep = nullptr, *rep_dev = nullptr;
+  uint64_t rep_seq = 0;
   Scratch();
   ~Scratch();
 };
+// the sum of kRedBlocks partials, stored with a new sequence number into the
+// report slot (no copy command, no event): returns the sequence number
+uint64_t report_sum(Scratch &S, const double *part, hipStream_t s);
+// waits (host spin, bounded) until report `seq` has arrived; returns its value
+double wait_report(Scratch &S, uint64_t seq, hipStream_t s);
 constexpr int kMaxDots = 64;
 
 // out[0] = <x,y>

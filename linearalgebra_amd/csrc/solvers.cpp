@@ -39,10 +39,6 @@ struct Work {
   DevBuf V, Z;  // GMRES bases
   la_dtype basis_dt = LA_F64;
   Scratch sc;
-  hipEvent_t ev = nullptr;  // marks the residual-norm copy of an iteration
-  ~Work() {
-    if (ev) (void)hipEventDestroy(ev);
-  }
 };
 
 thread_local std::unique_ptr<Work> tl_work;
@@ -55,7 +51,6 @@ Work &work_for(int64_t n) {
       tl_work->v64[i].alloc(sizeof(double) * std::max<int64_t>(n, 1));
       tl_work->v32[i].alloc(sizeof(float) * std::max<int64_t>(n, 1));
     }
-    LA_HIP(hipEventCreateWithFlags(&tl_work->ev, hipEventDisableTiming));
   }
   return *tl_work;
 }
@@ -148,16 +143,14 @@ CgResult pcg_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_dty
       dot_partials(n, vec, t, t, p_tt, s);
       p_check = p_tt;
     }
-    LA_HIP(hipMemcpyAsync(W.sc.host, p_check, sizeof(double) * kRedBlocks, hipMemcpyDeviceToHost, s));
-    LA_HIP(hipEventRecord(W.ev, s));
+    const uint64_t tk = report_sum(W.sc, p_check, s);
     // the next direction is queued before the stop test is read, so the
     // device runs it while the host waits; a converged run discards it
     // (x and r are final before it)
     apply_M(P, o, val, vec, r, z, n, s);
     dot_partials(n, vec, r, z, p_rz[cur ^ 1], s);
     cg_update_p(n, vec, p, z, p_rz[cur ^ 1], p_rz[cur], s);
-    LA_HIP(hipEventSynchronize(W.ev));
-    const double res = std::sqrt(sum_partials_host(W.sc.host));
+    const double res = std::sqrt(wait_report(W.sc, tk, s));
     R.iters = it;
     R.last_res = res;
     if (o.print_level == 1) std::fprintf(stderr, "%3d = %lg/%lg=%lg\n", it, res, nb, res / nb);
@@ -308,16 +301,14 @@ CgResult pcg_reliable(Mat &A, Prec *P, const la_solver_opts &o, la_dtype vd, con
     dot_partials(n, LA_F32, p, q, p_pq, s);
     cg_update_xr(n, LA_F32, xl, r, p, q, p_rz[cur], p_pq, p_rr, s);
     folded = false;
-    LA_HIP(hipMemcpyAsync(W.sc.host, p_rr, sizeof(double) * kRedBlocks, hipMemcpyDeviceToHost, s));
-    LA_HIP(hipEventRecord(W.ev, s));
+    const uint64_t tk = report_sum(W.sc, p_rr, s);
     // next direction queued before the test is read (see pcg_core); after a
     // replacement it stays the one built from the recurrence residual, a
     // valid search direction, while the residual itself is the true one
     apply_M(P, o, vd, LA_F32, r, z, n, s);
     dot_partials(n, LA_F32, r, z, p_rz[cur ^ 1], s);
     cg_update_p(n, LA_F32, p, z, p_rz[cur ^ 1], p_rz[cur], s);
-    LA_HIP(hipEventSynchronize(W.ev));
-    const double rr = std::sqrt(sum_partials_host(W.sc.host));
+    const double rr = std::sqrt(wait_report(W.sc, tk, s));
     R.iters = it;
     if (o.print_level == 1) std::fprintf(stderr, "%3d = %lg (recurrence, relative)\n", it, rr);
     if (!std::isfinite(rr)) break;

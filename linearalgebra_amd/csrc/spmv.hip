@@ -12,6 +12,7 @@
 //    of spmv.hpp:275-292, deterministic). Replaces SegSumSPMV and ALBUSSPMV.
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 
 #include "dev_util.hpp"
 #include "device.hpp"
@@ -292,6 +293,22 @@ void launch_seg(Mat &M, const T *av, const V *x, V *y, const V *z, A alpha, A be
 // gathered from the CSR values of the same precision.
 constexpr double kSellMaxFill = 1.15;  // padded entries / nnz allowed
 constexpr int kSellMaxWidth = 64;      // longest row allowed
+// Banded matrices (every |column - row| <= 32767, e.g. an RCM-ordered grid
+// operator) store each slot's column as a 16-bit offset from its row: 2
+// bytes per entry instead of 4 (a sixth of an FP64 SpMV's bytes, a third of
+// a BF16 one's). Padding is the offset -32768.
+constexpr int kSellPad16 = -32768;
+
+// a slot's column: int32 columns as stored (padding -1), 16-bit offsets
+// relative to the row
+__device__ __forceinline__ int sell_colof(const int *__restrict__ c, int64_t q, int, bool nt) {
+  return nt ? __builtin_nontemporal_load(c + q) : c[q];
+}
+__device__ __forceinline__ int sell_colof(const int16_t *__restrict__ c, int64_t q, int row,
+                                          bool nt) {
+  const int d = nt ? __builtin_nontemporal_load(c + q) : c[q];
+  return d == kSellPad16 ? -1 : row + d;
+}
 
 // W consecutive slice columns of one lane: all loads issued before any
 // product (the slice width is uniform across the wavefront)
@@ -299,9 +316,9 @@ constexpr int kSellMaxWidth = 64;      // longest row allowed
 // narrow slices -- width <= 8, the 7-point stencil -- run faster with
 // ordinary column loads, wide ones -- the 27-point stencil -- with
 // non-temporal ones)
-template <int W, bool NTC, typename T, typename V, typename A>
-__device__ __forceinline__ A sell_block(A acc, int k0, int64_t base,
-                                        const int *__restrict__ scol,
+template <int W, bool NTC, typename I, typename T, typename V, typename A>
+__device__ __forceinline__ A sell_block(A acc, int k0, int64_t base, int row,
+                                        const I *__restrict__ scol,
                                         const T *__restrict__ sval, const V *__restrict__ x) {
   using LT = decltype(ldm(sval, int64_t(0)));  // bf16 loads widen to float
   int c[W];
@@ -309,7 +326,7 @@ __device__ __forceinline__ A sell_block(A acc, int k0, int64_t base,
 #pragma unroll
   for (int e = 0; e < W; ++e) {
     const int64_t q = base + static_cast<int64_t>(k0 + e) * 64;
-    c[e] = NTC ? __builtin_nontemporal_load(scol + q) : scol[q];
+    c[e] = sell_colof(scol, q, row, NTC);
     v[e] = ldm(sval, q);
   }
   V xv[W];
@@ -323,10 +340,10 @@ __device__ __forceinline__ A sell_block(A acc, int k0, int64_t base,
   return acc;
 }
 
-template <typename T, typename V, typename A>
+template <typename I, typename T, typename V, typename A>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     int n, int64_t nslices, int uw, const int64_t *__restrict__ sp,
-    const int *__restrict__ sw, const int *__restrict__ scol, const T *__restrict__ sval,
+    const int *__restrict__ sw, const I *__restrict__ scol, const T *__restrict__ sval,
     const V *__restrict__ x, V *y, const V *z, A alpha, A beta, const T *__restrict__ rs) {
   const int64_t sl = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * (kBlock / 64) +
                      (threadIdx.x >> 6);
@@ -345,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     if (rs) sr = static_cast<A>(ldm(rs, row));
   }
   A acc = 0;
-#define LA_SB(WW, NT) acc = sell_block<WW, NT, T, V, A>(acc, k, base, scol, sval, x)
+#define LA_SB(WW, NT) acc = sell_block<WW, NT, I, T, V, A>(acc, k, base, row, scol, sval, x)
   int k = 0;
   if (w <= 8) {
     switch (w) {  // the whole row in one go
@@ -432,12 +449,41 @@ bool prepare_sell(Mat &M) {
   M.sell_len = len;
   M.sell_uniform_w = (ns && std::all_of(sw.begin(), sw.begin() + ns, [&](int w) { return w == sw[0]; }))
                          ? sw[0] : 0;
+  // 16-bit offsets when every column is within 32767 of its row
+  bool fits16 = true;
+  for (int64_t s = 0; s < ns && fits16; ++s)
+    for (int l = 0; l < 64; ++l) {
+      const int64_t r = s * 64 + l;
+      if (r >= n) break;
+      for (int k = 0; k < sw[s]; ++k) {
+        const int c = col[sp[s] + static_cast<int64_t>(k) * 64 + l];
+        if (c >= 0 && std::abs(static_cast<int64_t>(c) - r) > 32767) {
+          fits16 = false;
+          break;
+        }
+      }
+    }
+  if (const char *e = std::getenv("LAAMD_SELL16"))
+    if (e[0] == '0') fits16 = false;
+  M.sell_col16 = fits16;
   M.sell_ptr.alloc(sizeof(int64_t) * sp.size());
   M.sell_ptr.upload(sp.data(), sizeof(int64_t) * sp.size(), M.stream);
   M.sell_w.alloc(sizeof(int) * sw.size());
   M.sell_w.upload(sw.data(), sizeof(int) * sw.size(), M.stream);
-  M.sell_col.alloc(sizeof(int) * col.size());
-  M.sell_col.upload(col.data(), sizeof(int) * col.size(), M.stream);
+  if (fits16) {
+    std::vector<int16_t> c16(col.size());
+    for (int64_t s = 0; s < ns; ++s)
+      for (int64_t p = sp[s]; p < sp[s + 1]; ++p) {
+        const int64_t r = s * 64 + (p - sp[s]) % 64;
+        c16[p] = col[p] < 0 ? static_cast<int16_t>(kSellPad16)
+                            : static_cast<int16_t>(static_cast<int64_t>(col[p]) - r);
+      }
+    M.sell_col.alloc(sizeof(int16_t) * c16.size());
+    M.sell_col.upload(c16.data(), sizeof(int16_t) * c16.size(), M.stream);
+  } else {
+    M.sell_col.alloc(sizeof(int) * col.size());
+    M.sell_col.upload(col.data(), sizeof(int) * col.size(), M.stream);
+  }
   M.sell_src.alloc(sizeof(int) * src.size());
   M.sell_src.upload(src.data(), sizeof(int) * src.size(), M.stream);
   LA_HIP(hipStreamSynchronize(M.stream));
@@ -469,10 +515,15 @@ void launch_sell(Mat &M, const T *av, const V *x, V *y, const V *z, A alpha, A b
   const T *sv = sell_values<T>(M, DtOf<T>::v, av, s);
   constexpr int wpb = kBlock / 64;  // wavefronts per workgroup
   const unsigned grid = static_cast<unsigned>((M.sell_nslices + wpb - 1) / wpb);
-  hipLaunchKernelGGL((k_spmv_sell<T, V, A>), dim3(grid), dim3(kBlock), 0, s, M.rows,
-                     M.sell_nslices, M.sell_uniform_w, M.sell_ptr.as<int64_t>(),
-                     M.sell_w.as<int>(),
-                     M.sell_col.as<int>(), sv, x, y, z, alpha, beta, rs);
+  if (M.sell_col16)
+    hipLaunchKernelGGL((k_spmv_sell<int16_t, T, V, A>), dim3(grid), dim3(kBlock), 0, s, M.rows,
+                       M.sell_nslices, M.sell_uniform_w, M.sell_ptr.as<int64_t>(),
+                       M.sell_w.as<int>(), M.sell_col.as<int16_t>(), sv, x, y, z, alpha, beta,
+                       rs);
+  else
+    hipLaunchKernelGGL((k_spmv_sell<int, T, V, A>), dim3(grid), dim3(kBlock), 0, s, M.rows,
+                       M.sell_nslices, M.sell_uniform_w, M.sell_ptr.as<int64_t>(),
+                       M.sell_w.as<int>(), M.sell_col.as<int>(), sv, x, y, z, alpha, beta, rs);
   LA_HIP(hipGetLastError());
 }
 

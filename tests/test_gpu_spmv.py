@@ -33,8 +33,28 @@ def _cases():
           # narrow rows (at most 4 per row: two slices per wavefront), with
           # an odd slice count and a ragged last slice
           ("lower_laplace3d_12", lambda: orc.split_triangle(_o(la.gen.laplace3d(12)), False)),
-          ("bidiag_273", lambda: _bidiag(273))]
+          ("bidiag_273", lambda: _bidiag(273)),
+          # sliced ELL with 16-bit column offsets at their limits (+-32767),
+          # and one whose offsets do not fit (int32 columns)
+          ("offsets_at_16bit_limit", lambda: _band(70000, 32767)),
+          ("offsets_beyond_16bit", lambda: _band(70000, 40000))]
     return cs
+
+
+def _band(n, off):
+    """rows with columns i - off, i, i + off where they exist"""
+    rows, cols = [], []
+    for d in (-off, 0, off):
+        i = np.arange(max(0, -d), min(n, n - d))
+        rows.append(i)
+        cols.append(i + d)
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    o = np.lexsort((c, r))
+    r, c = r[o], c[o]
+    v = np.random.default_rng(off).standard_normal(len(r))
+    ai = np.zeros(n + 1, np.int64)
+    np.add.at(ai, r + 1, 1)
+    return orc.CSR(n, np.cumsum(ai).astype(np.int32), c.astype(np.int32), v)
 
 
 def _bidiag(n):
