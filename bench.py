@@ -48,11 +48,12 @@ HBM_PEAK_GBS = 8000.0
 ALG_NAMES = {0: "row", 1: "segsum", 2: "albus", 3: "auto", 4: "sell"}
 ALG_DESC = {0: "one wavefront per row", 2: "nnz-balanced segment kernel, 512 nnz per "
             "wavefront, + boundary-row fix-up",
-            4: "sliced ELL, 64-row column-major slices, one lane per row"}
+            4: "sliced ELL, 64-row column-major slices, one lane per row, 16-bit column "
+               "offsets where they fit"}
 ALG_KERNEL = {0: "k_spmv_row<G,double,double,double>",
               2: "k_spmv_seg<double,double,double> + k_spmv_seg_fix (one SpMV; duration "
                  "of the pair)",
-              4: "k_spmv_sell<double,double,double>"}
+              4: "k_spmv_sell<%s,double,double,double>"}
 
 
 def log(*a):
@@ -283,19 +284,27 @@ def bench_c4(la, torch, dev, stream, args):
     runs = {}
     # the configuration's two mixed forms: FP32 factors with the matrix in
     # FP32 or BF16; plus everything in BF16
+    # and with 6 Jacobi sweeps per triangle in place of the exact solves
+    P.set_sweeps(6)
+    J = dict(trsv_mode=la.TRSV_JACOBI)
     variants = [("fp64", dict(precision=la.PREC_FP64)),
                 ("mixed_f32", dict(precision=la.PREC_MIXED_F32, inner_tol=1e-5)),
                 ("mixed_bf16_matrix_f32_factors", dict(precision=la.PREC_MIXED_BF16_F32P,
                                                        inner_tol=1e-3)),
-                ("mixed_bf16", dict(precision=la.PREC_MIXED_BF16, inner_tol=1e-3))]
+                ("mixed_bf16", dict(precision=la.PREC_MIXED_BF16, inner_tol=1e-3)),
+                ("fp64_jacobi6", dict(precision=la.PREC_FP64, **J)),
+                ("mixed_f32_jacobi6", dict(precision=la.PREC_MIXED_F32, inner_tol=1e-5, **J)),
+                ("mixed_bf16_matrix_f32_factors_jacobi6",
+                 dict(precision=la.PREC_MIXED_BF16_F32P, inner_tol=1e-3, **J))]
     for name, kw in variants:
+        kw = dict(kw)
+        kw.setdefault("trsv_mode", mode)
         x = torch.zeros(A.n, dtype=torch.float64, device=dev)
         la.fgmres(mats[0], P, b, x, rel_tol=1e-10, restart=30, max_iters=3000,
-                  trsv_mode=mode, **kw)  # untimed: builds precision copies
+                  **kw)  # untimed: builds precision copies
         x.zero_()
         torch.cuda.synchronize()
-        r = la.fgmres(mats[0], P, b, x, rel_tol=1e-10, restart=30, max_iters=3000,
-                      trsv_mode=mode, **kw)
+        r = la.fgmres(mats[0], P, b, x, rel_tol=1e-10, restart=30, max_iters=3000, **kw)
         runs[name] = {"seconds": round(r["seconds"], 4), "iters": r["iters"],
                       "outer": r["outer_iters"], "rel_residual": float("%.3e" % r["rel_residual"]),
                       "converged": r["converged"]}
@@ -304,15 +313,16 @@ def bench_c4(la, torch, dev, stream, args):
     r = torch.tensor(la.gen.uniform(A.n, 5), device=dev)
     z = torch.zeros_like(r)
     for val, vn in [(la.F64, "f64"), (la.F32, "f32")]:
-        rr = r if val == la.F64 else r.float()
-        zz = z if val == la.F64 else z.float()
-        P.apply(rr, zz, mode=mode, val=val)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(5):
-            P.apply(rr, zz, mode=mode, val=val)
-        torch.cuda.synchronize()
-        ap[vn] = round((time.perf_counter() - t0) / 5 * 1e3, 3)
+        for md, mn in [(mode, args.trsv), (la.TRSV_JACOBI, "jacobi6")]:
+            rr = r if val == la.F64 else r.float()
+            zz = z if val == la.F64 else z.float()
+            P.apply(rr, zz, mode=md, val=val)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                P.apply(rr, zz, mode=md, val=val)
+            torch.cuda.synchronize()
+            ap["%s_%s" % (mn, vn)] = round((time.perf_counter() - t0) / 5 * 1e3, 3)
     out = {"config": "c4 aniso27 %d^3" % grid, "n": A.n, "nnz": A.nnz, "spmv": tab,
            "auto_runs": ALG_NAMES[mats[0].resolve(la.SPMV_AUTO)],
            "ilu1": {"nnz": int(len(faj)), "levels_L_U": P.levels(), "gen_s": round(t_gen, 2),
@@ -322,9 +332,12 @@ def bench_c4(la, torch, dev, stream, args):
                                        "numeric_s": round(t_gnum, 4),
                                        "bitwise_equal_host": same}},
            "trsv_mode": args.trsv, "gmres30": runs}
-    f64 = runs["fp64"]["seconds"]
-    for k in ("mixed_f32", "mixed_bf16_matrix_f32_factors", "mixed_bf16"):
-        out["speedup_%s_vs_fp64" % k] = round(f64 / runs[k]["seconds"], 3)
+    sec = {k: v["seconds"] for k, v in runs.items()}
+    out["speedup_vs_plain_fp64"] = {k: round(sec["fp64"] / sec[k], 3)
+                                    for k in sec if k.startswith("mixed")}
+    out["speedup_vs_fp64_same_trsv"] = {
+        k: round(sec["fp64_jacobi6" if "jacobi" in k else "fp64"] / sec[k], 3)
+        for k in sec if k.startswith("mixed")}
     print(json.dumps(out))
 
 
@@ -380,6 +393,7 @@ def main():
     # parity check of the timed kernel against the FP64 oracle (cheap)
     from oracle import oracle as orc
     chosen = mats[0].resolve(la.SPMV_AUTO)
+    sinfo = mats[0].sell_info()
     mats[0].spmv(x, y, alg=la.SPMV_AUTO)
     ref = orc.spmv(orc.CSR(n, A.ai, A.aj, A.av), x.cpu().numpy())
     rel = float(np.linalg.norm(y.cpu().numpy() - ref) / np.linalg.norm(ref))
@@ -471,13 +485,16 @@ def main():
         "config": {"workload": "3-D 7-point Laplacian 128^3, FP64 CSR int32, RCM-ordered; "
                                "y = A x (%s)" % ALG_DESC[chosen],
                    "n": n, "nnz": nnz, "bytes_per_spmv": B,
+                   "sell": {"col16": sinfo["col16"],
+                            "fill": round(sinfo["slots"] / nnz, 4) if sinfo["built"] else None},
                    "pct_hbm_peak": round(100 * gbs_kernel / HBM_PEAK_GBS, 1),
                    "wall_GBs": round(gbs_wall, 1), "host_prep": prep,
                    "other_spmv": extra},
         "roofline": {"bound": "hbm", "achieved": round(gbs_kernel, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs_kernel / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": ALG_KERNEL[chosen],
+                     "kernel": (ALG_KERNEL[chosen] % ("short" if sinfo["col16"] else "int")
+                                if chosen == 4 else ALG_KERNEL[chosen]),
                      "avg_launch_us": round(avg_ms * 1e3, 3)},
         "cpu_baseline": cpu,
         "pcg": pcg,

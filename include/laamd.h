@@ -180,6 +180,10 @@ int la_spmv_preprocess(la_mat_t A, la_spmv_alg alg);
  * ALBUS's kernels; an empty matrix runs ROW. Builds what the choice needs.
  * No counterpart in the reference (its algorithm is a template argument). */
 int la_spmv_resolve(la_mat_t A, la_spmv_alg alg, int *chosen);
+/* The sliced-ELL copy (built on demand): *built 1 if the matrix has one,
+ * *col16 1 if its columns are 16-bit offsets from the row, *slots the
+ * stored slots (nnz plus padding). No counterpart in the reference. */
+int la_mat_sell_info(la_mat_t A, int *built, int *col16, int64_t *slots);
 /* y = alpha*op(A)*x + beta*y (device pointers), the functor call
  * operator()(size, base, ai, aj, av, b, x, alpha, beta) of
  * sparse_mat_op/spmv.hpp:16-29 with b -> x and x -> y, and

@@ -125,6 +125,12 @@ class DeviceMatrix:
     def preprocess(self, alg=SPMV_ALBUS):
         check(lib().la_spmv_preprocess(self.h, alg))
 
+    def sell_info(self):
+        """{"built", "col16", "slots"} of the sliced-ELL copy (la_mat_sell_info)."""
+        b, c16, sl = C.c_int(), C.c_int(), C.c_int64()
+        check(lib().la_mat_sell_info(self.h, C.byref(b), C.byref(c16), C.byref(sl)))
+        return {"built": bool(b.value), "col16": bool(c16.value), "slots": sl.value}
+
     def resolve(self, alg=SPMV_AUTO):
         """The kernel family `alg` runs on for this matrix (SPMV_* code)."""
         out = C.c_int(0)

@@ -80,6 +80,7 @@ _SIGS = {
     "la_mat_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i64)],
     "la_spmv_preprocess": [_vp, _i],
     "la_spmv_resolve": [_vp, _i, C.POINTER(_i)],
+    "la_mat_sell_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i64)],
     "la_spmv": [_vp, _i, _i, _i, _i, _d, _vp, _d, _vp],
     "la_spmv_axpby": [_vp, _i, _i, _i, _i, _d, _vp, _d, _vp, _vp],
     "la_spmv_transpose": [_vp, _i, _i, _d, _vp, _d, _vp],

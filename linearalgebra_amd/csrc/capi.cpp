@@ -335,6 +335,15 @@ int la_spmv_resolve(la_mat_t A, la_spmv_alg alg, int *chosen) {
   *chosen = static_cast<int>(spmv_resolve(*A->m, alg));
   LA_CATCH
 }
+int la_mat_sell_info(la_mat_t A, int *built, int *col16, int64_t *slots) {
+  LA_TRY
+  LA_CHECK_ARG(A, "null handle");
+  const bool ok = spmv_prepare_sell(*A->m);
+  if (built) *built = ok ? 1 : 0;
+  if (col16) *col16 = ok && A->m->sell_col16 ? 1 : 0;
+  if (slots) *slots = ok ? A->m->sell_len : 0;
+  LA_CATCH
+}
 int la_spmv(la_mat_t A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, double alpha,
             const void *x, double beta, void *y) {
   LA_TRY

@@ -239,3 +239,11 @@ def test_resolve_names_the_kernel(dev):
     assert pl.resolve(la.SPMV_SELL) == la.SPMV_ALBUS
     em = la.DeviceMatrix(as_lib_csr(orc.CSR(7, np.zeros(8), [], [])))
     assert em.resolve(la.SPMV_AUTO) == la.SPMV_ROW
+    # the sliced-ELL copy: 16-bit column offsets exactly when they fit
+    A = la.gen.laplace3d(12)
+    info = st.sell_info()
+    assert info["built"] and info["col16"] and info["slots"] >= A.nnz
+    assert la.DeviceMatrix(as_lib_csr(_band(70000, 32767))).sell_info()["col16"]
+    wide = la.DeviceMatrix(as_lib_csr(_band(70000, 32768))).sell_info()
+    assert wide["built"] and not wide["col16"]
+    assert not pl.sell_info()["built"]
