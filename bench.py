@@ -18,7 +18,8 @@ int32 indices, RCM-reordered on the host.
     through reliable updates), each with exact (sync-free) triangular
     solves and with 3 Jacobi sweeps per triangle.
   * roofline: the SpMV kernel against the 8 TB/s HBM peak, its average
-    launch duration from HIP events on the stream it runs on.
+    launch duration from one pair of HIP events around the K timed launches
+    on the stream they run on (so gaps between launches count too).
   * cpu_baseline: the oracle's OpenMP SpMV (a restatement of the reference's
     ParallelSPMV) on this host's cores, timed on the same matrix for a
     bounded ~10 s sample.
@@ -77,22 +78,22 @@ def build_c2(la, grid=128, rcm=True):
 
 
 def time_spmv(la, torch, mats, x, y, steps, warmup, alg, val, acc, stream):
-    """Per-launch HIP-event durations (ms) and the wall time of the K
-    launches (ms), rotating over `mats`."""
+    """Average launch duration (ms) of K back-to-back launches, from one pair
+    of HIP events around them on the stream they run on (launch gaps
+    included), and the host wall time of the K launches (ms); rotating over
+    `mats`."""
     for i in range(warmup):
         mats[i % len(mats)].spmv(x, y, alg=alg, val=val, acc=acc)
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(steps)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    e0.record(stream)
     for i in range(steps):
-        ev[i][0].record(stream)
         mats[i % len(mats)].spmv(x, y, alg=alg, val=val, acc=acc)
-        ev[i][1].record(stream)
+    e1.record(stream)
     torch.cuda.synchronize()
     wall_ms = (time.perf_counter() - t0) * 1e3
-    durs = [a.elapsed_time(b) for a, b in ev]
-    return durs, wall_ms
+    return [e0.elapsed_time(e1) / steps], wall_ms
 
 
 def cpu_baseline(A, seconds=10.0):

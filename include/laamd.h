@@ -105,9 +105,16 @@ int la_permute(int n, int base, const int *ai, const int *aj,
                const double *av, const int *iperm, const int *perm, int *pai,
                int *paj, double *pav);
 /* reordering/Reordering.cpp:37-136 SerialCM (reverse Cuthill-McKee).
- * iperm[new] = old + base; perm its inverse. */
+ * iperm[new] = old + base; perm its inverse. Runs with a team of host
+ * threads (LAAMD_HOST_THREADS, else the hardware's, at most 16) on
+ * structurally symmetric patterns, serially otherwise; the result is the
+ * serial definition's, bit for bit, either way. */
 int la_rcm(int n, int base, const int *ai, const int *aj, int *iperm,
            int *perm);
+/* la_rcm with an explicit thread count (<= 0: the default); the same
+ * result for every count. */
+int la_rcm_threads(int n, int base, const int *ai, const int *aj, int nthreads,
+                   int *iperm, int *perm);
 /* mkl_wrapper/incomplete_lu.cpp:153-275 incomplete_lu_k::symbolic_factorize.
  * Two-phase (faj NULL: fai and *nnz only). first_upper[n] are 0-based
  * positions of the first column > i. */

@@ -476,9 +476,12 @@ def permute(A, iperm, perm):
     return CSR(A.n, pai, paj, pav)
 
 
-def rcm(A):
+def rcm(A, nthreads=0):
+    """Reverse Cuthill-McKee (reordering/Reordering.cpp:37-136), bit-exact
+    with the serial definition for any thread count (0: the default)."""
     iperm, perm = np.zeros(A.n, np.int32), np.zeros(A.n, np.int32)
-    check(lib().la_rcm(A.n, A.base, _p(A.ai), _p(A.aj), _p(iperm), _p(perm)))
+    check(lib().la_rcm_threads(A.n, A.base, _p(A.ai), _p(A.aj), int(nthreads), _p(iperm),
+                               _p(perm)))
     return iperm, perm
 
 

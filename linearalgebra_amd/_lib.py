@@ -60,6 +60,7 @@ _SIGS = {
     "la_transpose": [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "la_permute": [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "la_rcm": [_i, _i, _vp, _vp, _vp, _vp],
+    "la_rcm_threads": [_i, _i, _vp, _vp, _i, _vp, _vp],
     "la_iluk_symbolic": [_i, _i, _vp, _vp, _i, _vp, _vp, _vp, C.POINTER(_i64)],
     "la_iluk_numeric": [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "la_icc_symbolic": [_i, _i, _vp, _vp, _vp, _i, _vp, _vp, C.POINTER(_i64)],

@@ -173,6 +173,13 @@ int la_rcm(int n, int base, const int *ai, const int *aj, int *iperm, int *perm)
   rcm(n, base, ai, aj, iperm, perm);
   LA_CATCH
 }
+int la_rcm_threads(int n, int base, const int *ai, const int *aj, int nthreads, int *iperm,
+                   int *perm) {
+  LA_TRY
+  LA_CHECK_ARG(n >= 0 && ai && iperm && perm, "null argument");
+  rcm(n, base, ai, aj, iperm, perm, nthreads);
+  LA_CATCH
+}
 
 int la_iluk_symbolic(int n, int base, const int *ai, const int *aj, int level, int *fai,
                      int *faj, int *first_upper, int64_t *nnz) {

@@ -269,7 +269,7 @@ int pseudo_peripheral(int n, int base, const int *ai, const int *aj,
 // taken in ascending root order with members ascending. Each component is
 // ordered by BFS from a pseudo-peripheral node, children of a node sorted
 // by (degree, index); the whole order is then reversed.
-void rcm(int n, int base, const int *ai, const int *aj, int *iperm, int *perm) {
+void rcm_serial(int n, int base, const int *ai, const int *aj, int *iperm, int *perm) {
   std::vector<int> deg(n), parent(n);
   for (int i = 0; i < n; ++i) { deg[i] = ai[i + 1] - ai[i]; parent[i] = i; }
   for (int i = 0; i < n; ++i)

@@ -22,7 +22,14 @@ void transpose(int rows, int cols, int base, const int *ai, const int *aj,
 void permute(int n, int base, const int *ai, const int *aj, const double *av,
              const int *iperm, const int *perm, int *pai, int *paj,
              double *pav);
-void rcm(int n, int base, const int *ai, const int *aj, int *iperm, int *perm);
+// RCM, bit-exact with the reference's SerialCM: with a team of host threads
+// (nthreads <= 0: LAAMD_HOST_THREADS, else the hardware's, at most 16) for
+// structurally symmetric patterns, serially otherwise
+void rcm(int n, int base, const int *ai, const int *aj, int *iperm, int *perm,
+         int nthreads = 0);
+void rcm_serial(int n, int base, const int *ai, const int *aj, int *iperm, int *perm);
+bool pattern_symmetric(int n, int base, const int *ai, const int *aj);
+int rcm_default_threads();
 void iluk_symbolic(int n, int base, const int *ai, const int *aj, int level,
                    std::vector<int> &fai, std::vector<int> &faj,
                    std::vector<int> &first_upper);

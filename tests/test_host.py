@@ -193,3 +193,58 @@ def test_uniform_lognormal_deterministic():
     np.testing.assert_array_equal(u, la.gen.uniform(100000, 42))
     g = la.gen.lognormal(200000, 7, 0.0, 1.0)
     assert abs(np.log(g).mean()) < 0.01 and abs(np.log(g).std() - 1) < 0.01
+
+
+def _sym_graphs():
+    """structurally symmetric patterns large enough (n >= 4096) for the
+    threaded RCM, plus an unsymmetric one (serial path)"""
+    import scipy.sparse as sp
+
+    def csr(M, base=0):
+        M = sp.csr_matrix(M)
+        M.sort_indices()
+        return orc.CSR(M.shape[0], (M.indptr + base).astype(np.int32),
+                       (M.indices + base).astype(np.int32), M.data.astype(float))
+
+    def lap(g):
+        A = la.gen.laplace3d(g)
+        return sp.csr_matrix((A.av, A.aj, A.ai), shape=(A.n, A.n))
+
+    def scrambled(M, seed):
+        p = np.random.default_rng(seed).permutation(M.shape[0])
+        return M[p][:, p]
+
+    def random_sym(n, deg, seed):
+        rng = np.random.default_rng(seed)
+        r = rng.integers(0, n, n * deg)
+        c = rng.integers(0, n, n * deg)
+        M = sp.coo_matrix((np.ones(len(r)), (r, c)), shape=(n, n)).tocsr()
+        return M + M.T + sp.eye(n)
+
+    def components():
+        # three grids, two isolated nodes, a pair, a path, scrambled together
+        blocks = [lap(14), lap(9), sp.eye(2), sp.csr_matrix(np.array([[1.0, 1], [1, 1]])),
+                  sp.diags([np.ones(499), np.ones(500), np.ones(499)], [-1, 0, 1]), lap(10)]
+        return scrambled(sp.block_diag(blocks).tocsr(), 3)
+
+    return [("laplace3d_20_scrambled", lambda: csr(scrambled(lap(20), 1))),
+            ("poisson2d_120", lambda: csr(sp.csr_matrix(
+                (lambda A: (A.av, A.aj, A.ai))(la.gen.poisson2d(120)), shape=(14400, 14400)))),
+            ("random_sym_20000", lambda: csr(random_sym(20000, 3, 7))),
+            ("components", lambda: csr(components())),
+            ("components_base1", lambda: csr(components(), base=1)),
+            ("unsymmetric_6000", lambda: csr(sp.random(6000, 6000, density=5e-4, random_state=4)
+                                             + sp.eye(6000)))]
+
+
+@pytest.mark.parametrize("name,make", _sym_graphs(), ids=[g[0] for g in _sym_graphs()])
+def test_rcm_threads_bit_exact(name, make):
+    """The threaded RCM equals the oracle's serial restatement for every
+    thread count (components, isolated nodes, base 1; an unsymmetric
+    pattern takes the serial code)."""
+    A = make()
+    i1, p1 = orc.rcm(A)
+    for nt in (1, 2, 3, 8):
+        i2, p2 = la.rcm(as_lib_csr(A), nthreads=nt)
+        np.testing.assert_array_equal(i1, i2, err_msg="threads %d" % nt)
+        np.testing.assert_array_equal(p1, p2, err_msg="threads %d" % nt)
