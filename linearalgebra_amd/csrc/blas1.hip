@@ -616,12 +616,6 @@ void dot_partials(int64_t n, la_dtype vec, const void *x, const void *y, double 
   LA_HIP(hipGetLastError());
 }
 
-double sum_partials_host(const double *h) {
-  double t = 0;
-  for (int b = 0; b < kRedBlocks; ++b) t += h[b];
-  return t;
-}
-
 void cg_update_xr(int64_t n, la_dtype vec, void *x, void *r, const void *p, const void *q,
                   const double *part_rz, const double *part_pq, double *part_rr, hipStream_t s) {
   if (al16(x) && al16(r) && al16(p) && al16(q)) {

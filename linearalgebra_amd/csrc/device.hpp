@@ -198,11 +198,10 @@ void copy_vec(int64_t n, la_dtype src_dt, const void *src, la_dtype dst_dt, void
 void fill_zero(int64_t n, la_dtype vec, void *x, hipStream_t s);
 
 // CG step kernels. Dot products stay as kRedBlocks per-block partials; the
-// consuming kernel sums them itself (same order in every block), the host
-// sums them when it needs the value (sum_partials_host).
+// consuming kernel sums them itself (same order in every block), and
+// report_sum hands the host the same sum when it needs the value.
 void dot_partials(int64_t n, la_dtype vec, const void *x, const void *y, double *partials,
                   hipStream_t s);
-double sum_partials_host(const double *h);
 // x += a p, r -= a q, a = <r,z>/<p,q>; partials of <r,r> into part_rr
 void cg_update_xr(int64_t n, la_dtype vec, void *x, void *r, const void *p, const void *q,
                   const double *part_rz, const double *part_pq, double *part_rr, hipStream_t s);

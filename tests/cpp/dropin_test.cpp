@@ -107,6 +107,21 @@ int main() {
   ok = pcg.solve(rhs.data(), sol.data());
   expect(ok && rel(sol, b) < 1e-7, "mkl_pcg_solver + IC(0)", rel(sol, b));
   expect(pcg.last_result().rel_residual < 1e-10, "PCG true residual", pcg.last_result().rel_residual);
+  // this build's extensions on the same solver: Jacobi-sweep triangular
+  // solves and mixed BF16 precision with reliable updates, still to an FP64
+  // true residual of 1e-10
+  prec->set_trsv_mode(LA_TRSV_JACOBI);
+  prec->set_sweeps(3);
+  pcg.set_precision(LA_PREC_MIXED_BF16);
+  pcg.set_reliable_delta(0.1);
+  std::fill(sol.begin(), sol.end(), 0.0);
+  ok = pcg.solve(rhs.data(), sol.data());
+  expect(ok && pcg.last_result().rel_residual < 1e-10, "PCG mixed BF16 + Jacobi sweeps",
+         pcg.last_result().rel_residual);
+  expect(pcg.last_result().outer_iters > 1, "reliable updates replaced the residual",
+         pcg.last_result().outer_iters);
+  pcg.set_precision(LA_PREC_FP64);
+  prec->set_trsv_mode(LA_TRSV_SYNCFREE);
 
   // ILU(1)-FGMRES through the factory, the "gmres+iluk" registration of
   // execs/test1.cpp:61-77

@@ -161,3 +161,33 @@ def test_pcg_with_sweeps_reaches_tolerance(dev, precision):
     bh = b.cpu().numpy()
     assert np.linalg.norm(bh - r) <= 1.01e-10 * np.linalg.norm(bh)
     assert res["iters"] <= 2 * ex["iters"]
+
+
+@pytest.mark.parametrize("k", [0, 1, 4])
+def test_tiny_and_long_row_triangles(dev, k):
+    """1x1 triangles (no entries: diagonal scaling or a copy), and a
+    triangle with a row longer than 64 entries, which has no sliced-ELL
+    form (the last sweep then scales in a separate pass)."""
+    import torch
+    one = orc.CSR(1, np.array([0, 0], np.int32), np.zeros(0, np.int32), np.zeros(0))
+    b1 = np.array([3.0])
+    x = torch.zeros(1, dtype=torch.float64, device=dev)
+    T = la.TriangularSolve(as_lib_csr(one), np.array([2.0]), upper=True).set_sweeps(k)
+    T.solve(torch.tensor(b1, device=dev), x, mode=la.TRSV_JACOBI)
+    assert x.item() == 1.5
+    T = la.TriangularSolve(as_lib_csr(one), None, upper=False).set_sweeps(k)
+    T.solve(torch.tensor(b1, device=dev), x, mode=la.TRSV_JACOBI)
+    assert x.item() == 3.0
+    # strict lower triangle: row n-1 holds columns 0..99 (> 64 entries)
+    n = 300
+    rng = np.random.default_rng(11)
+    rows = [np.arange(max(0, i - 2), i) for i in range(n - 1)] + [np.arange(100)]
+    ai = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    aj = np.concatenate(rows).astype(np.int32)
+    L = orc.CSR(n, ai, aj, rng.uniform(-0.3, 0.3, len(aj)))
+    d = rng.uniform(1.0, 2.0, n)
+    b = la.gen.uniform(n, 12)
+    Tl = la.TriangularSolve(as_lib_csr(L), d, upper=False).set_sweeps(k)
+    xt = torch.zeros(n, dtype=torch.float64, device=dev)
+    Tl.solve(torch.tensor(b, device=dev), xt, mode=la.TRSV_JACOBI)
+    assert rel_err(xt.cpu().numpy(), orc.trsv_jacobi(L, d, b, k)) <= 1e-12
