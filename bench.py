@@ -69,12 +69,15 @@ def build_c2(la, grid=128, rcm=True):
     t0 = time.time()
     A = la.gen.laplace3d(grid)
     t_gen = time.time() - t0
-    t0 = time.time()
+    t_rcm = t_perm = 0.0
     if rcm:
+        t0 = time.time()
         ip, p = la.rcm(A)
+        t_rcm = time.time() - t0
+        t0 = time.time()
         A = la.permute(A, ip, p)
-    t_rcm = time.time() - t0
-    return A, {"gen_s": round(t_gen, 3), "rcm_s": round(t_rcm, 3)}
+        t_perm = time.time() - t0
+    return A, {"gen_s": round(t_gen, 3), "rcm_s": round(t_rcm, 3), "permute_s": round(t_perm, 3)}
 
 
 def time_spmv(la, torch, mats, x, y, steps, warmup, alg, val, acc, stream):
