@@ -27,9 +27,10 @@ int32 indices, RCM-reordered on the host.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4]
        [--no-pcg] [--no-cpu] [--pmc-probe] [--trsv level|syncfree] [--c4-grid G]
 
---config c3 (power-law SpMV in FP64/FP32/BF16) and --config c4 (27-point
-160^3: SpMV, ILU(1)-GMRES(30) FP64 vs mixed precision) print their own
-JSON records; the default (c2) prints the bench line.
+--config c3 (power-law SpMV in FP64/FP32/BF16), --config c4 (27-point
+160^3: SpMV, ILU(1)-GMRES(30) FP64 vs mixed precision) and --config c5
+(the adjoint of a 512^2 diffusion solve) print their own JSON records; the
+default (c2) prints the bench line.
 """
 import argparse
 import json
@@ -345,6 +346,89 @@ def bench_c4(la, torch, dev, stream, args):
     print(json.dumps(out))
 
 
+def bench_c5(la, torch, dev, stream, args):
+    """Adjoint of x = A(theta)^-1 b, 2-D 5-point edge-conductivity diffusion
+    on a 512^2 grid, theta ~ lognormal(0, 0.5) (seeded): loss ||x - x*||^2
+    with x* = A(theta*)^-1 b; one forward IC(0)-PCG solve, the transposed
+    (adjoint) solve and the value gradient, timed per variant, with each
+    gradient's relative error against an FP64 CPU reference (sparse direct
+    LU and the closed form dL/dtheta_e = -(lambda_i - lambda_j)(x_i - x_j))."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    from linearalgebra_amd.autograd import SparseSystem, edge_values, sparse_solve
+    nx = 512
+    ne = 2 * nx * nx - 2 * nx
+    theta = la.gen.lognormal(ne, 5, 0.0, 0.5)
+    theta_star = la.gen.lognormal(ne, 55, 0.0, 0.5)
+    A0, ei, ej, vm = la.gen.diffusion2d(nx, nx, np.zeros(ne))
+    bh = la.gen.uniform(nx * nx, 505)
+    vm_t = torch.tensor(vm, dtype=torch.int64, device=dev)
+    base = torch.tensor(A0.av, device=dev)
+    # FP64 CPU reference gradient
+    t0 = time.time()
+    A, _, _, _ = la.gen.diffusion2d(nx, nx, theta)
+    As, _, _, _ = la.gen.diffusion2d(nx, nx, theta_star)
+
+    def lu(M):
+        return spl.splu(sp.csr_matrix((M.av, M.aj, M.ai), shape=(M.n, M.n)).tocsc(),
+                        permc_spec="MMD_AT_PLUS_A")
+    LA = lu(A)
+    x_ref = LA.solve(bh)
+    lam_ref = LA.solve(2 * (x_ref - lu(As).solve(bh)))
+    g_ref = -(lam_ref[ei] - lam_ref[ej]) * (x_ref[ei] - x_ref[ej])
+    t_ref = time.time() - t0
+    J = dict(trsv_mode=la.TRSV_JACOBI)
+    variants = [("fp64", dict(precision=la.PREC_FP64)),
+                ("mixed_f32", dict(precision=la.PREC_MIXED_F32)),
+                ("fp64_jacobi3", dict(precision=la.PREC_FP64, **J)),
+                ("mixed_f32_jacobi3", dict(precision=la.PREC_MIXED_F32, **J)),
+                ("mixed_bf16_jacobi3", dict(precision=la.PREC_MIXED_BF16, **J))]
+    runs = {}
+    for name, kw in variants:
+        system = SparseSystem(A0, kind="ic", level=0, rel_tol=1e-11, max_iters=20000,
+                              adjoint_rel_tol=1e-10, **kw)
+        system.P.set_sweeps(3)
+
+        def once():
+            b = torch.tensor(bh, device=dev, requires_grad=True)
+            th = torch.tensor(theta, device=dev, requires_grad=True)
+            with torch.no_grad():
+                xs = sparse_solve(edge_values(torch.tensor(theta_star, device=dev), base, vm_t),
+                                  b.detach(), system)
+            x = sparse_solve(edge_values(th, base, vm_t), b, system)
+            loss = ((x - xs) ** 2).sum()
+            loss.backward()
+            return th.grad, b.grad
+        try:
+            once()  # untimed: precision copies, Jacobi forms, transposed plans
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g, gb = once()
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+        except la.LaError as e:  # reported, not hidden: a precision too coarse
+            runs[name] = {"converged": False, "error": str(e)}
+            log("c5", name, runs[name])
+            continue
+        g, gb = g.cpu().numpy(), gb.cpu().numpy()
+        runs[name] = {
+            "seconds_two_forward_one_adjoint": round(el, 4),
+            "forward_iters": system.last["forward"]["iters"],
+            "adjoint_iters": system.last["adjoint"]["iters"],
+            "grad_theta_rel_err": float("%.3e" % (np.linalg.norm(g - g_ref) /
+                                                 np.linalg.norm(g_ref))),
+            "grad_b_rel_err": float("%.3e" % (np.linalg.norm(gb - lam_ref) /
+                                             np.linalg.norm(lam_ref))),
+            "converged": True}
+        log("c5", name, runs[name])
+    sec = {k: v["seconds_two_forward_one_adjoint"] for k, v in runs.items()
+           if "seconds_two_forward_one_adjoint" in v}
+    print(json.dumps({"config": "c5 adjoint 2-D diffusion 512^2", "n": nx * nx,
+                      "edges": ne, "cpu_reference_s": round(t_ref, 2), "runs": runs,
+                      "speedup_vs_plain_fp64": {k: round(sec["fp64"] / sec[k], 3)
+                                                for k in sec if k != "fp64"}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -375,8 +459,10 @@ def main():
         return bench_c3(la, torch, dev, stream, args)
     if args.config == "c4":
         return bench_c4(la, torch, dev, stream, args)
+    if args.config == "c5":
+        return bench_c5(la, torch, dev, stream, args)
     if args.config != "c2":
-        raise SystemExit("--config is one of c2 (the bench line), c3, c4")
+        raise SystemExit("--config is one of c2 (the bench line), c3, c4, c5")
     A, prep = build_c2(la)
     n, nnz = A.n, A.nnz
     log("matrix n=%d nnz=%d prep=%s" % (n, nnz, prep))
