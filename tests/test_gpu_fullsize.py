@@ -11,7 +11,12 @@
   above), FP32 values (2e-6 of (|A||x|)_i) and BF16 values (2e-2), FP32
   accumulation, against the FP64 oracle.
 * 27-point anisotropic 160^3, n = 4,096,000, nnz = 109,215,352: SpMV FP64
-  and the ILU(1) apply (both modes) against the oracle, 1e-12.
+  (FP32/BF16 values at the storage bounds above) and the ILU(1) apply (both
+  modes) against the oracle, 1e-12.
+* The SpMV forms include the sliced-ELL kernel with 16-bit column offsets
+  that AUTO (and the bench line) runs on both grids; 128^3 also checks the
+  3-sweep Jacobi IC(0) apply against the oracle's sweeps (1e-12) and mixed
+  BF16 PCG with it to a 1e-10 FP64 true residual, recomputed on the host.
 """
 import json
 import os
@@ -63,8 +68,11 @@ def _c2():
 
 def test_c2_spmv_full(dev):
     A = _c2()
-    _check_spmv(dev, A, [la.SPMV_ROW, la.SPMV_SEGSUM, la.SPMV_ALBUS],
-                [(la.F64, 1e-12), (la.F32, 2e-6), (la.BF16, 2e-2)])
+    d = _check_spmv(dev, A, [la.SPMV_ROW, la.SPMV_SEGSUM, la.SPMV_ALBUS, la.SPMV_SELL,
+                             la.SPMV_AUTO],
+                    [(la.F64, 1e-12), (la.F32, 2e-6), (la.BF16, 2e-2)])
+    # the bench line's kernel: sliced ELL with 16-bit column offsets
+    assert d.resolve(la.SPMV_AUTO) == la.SPMV_SELL and d.sell_info()["col16"]
 
 
 def test_c2_ic0_apply_and_pcg_full(dev):
@@ -107,7 +115,10 @@ def test_c4_spmv_and_ilu1_apply_full(dev):
     import torch
     A = la.gen.aniso27(160)
     assert (A.n, A.nnz) == (4_096_000, 109_215_352)
-    _check_spmv(dev, A, [la.SPMV_ALBUS], [(la.F64, 1e-12)])
+    d = _check_spmv(dev, A, [la.SPMV_ALBUS, la.SPMV_AUTO],
+                    [(la.F64, 1e-12), (la.F32, 2e-6), (la.BF16, 2e-2)])
+    assert d.resolve(la.SPMV_AUTO) == la.SPMV_SELL and d.sell_info()["col16"]
+    del d
     F = la.iluk(A, 1)
     P = la.Preconditioner(F, "ilu")
     r = la.gen.uniform(A.n, 6)
@@ -119,3 +130,32 @@ def test_c4_spmv_and_ilu1_apply_full(dev):
         err = rel_err(z.cpu().numpy(), zr)
         print("ILU(1) apply mode=%d rel err %.3e" % (mode, err), flush=True)
         assert err < 1e-12
+
+
+def test_c2_jacobi_apply_and_mixed_pcg_full(dev):
+    """128^3: the 3-sweep IC(0) apply against the oracle's sweeps (1e-12),
+    and mixed BF16 PCG with those sweeps (reliable updates) to a 1e-10 FP64
+    true residual, checked again on the host with the oracle's SpMV."""
+    import torch
+    A = _c2()
+    R, ok = la.icc(la.split_triangle(A, upper=True), 0)
+    P = la.Preconditioner(R, "ic").set_sweeps(3)
+    Ro = _o(R)
+    Rl, D, Ru = orc.split_ldu(Ro)
+    r = la.gen.uniform(A.n, 8)
+    ref = orc.trsv_jacobi(Ru, D, orc.trsv_jacobi(orc.transpose(Ru), D, r, 3), 3)
+    rt = torch.tensor(r, device=dev)
+    z = torch.zeros_like(rt)
+    P.apply(rt, z, mode=la.TRSV_JACOBI)
+    err = rel_err(z.cpu().numpy(), ref)
+    print("Jacobi-3 IC(0) apply rel err %.3e" % err, flush=True)
+    assert err < 1e-12
+    b = torch.tensor(la.gen.uniform(A.n, 2024), device=dev)
+    x = torch.zeros_like(b)
+    res = la.pcg(la.DeviceMatrix(A), P, b, x, rel_tol=1e-10, max_iters=5000,
+                 precision=la.PREC_MIXED_BF16, trsv_mode=la.TRSV_JACOBI)
+    print(res, flush=True)
+    assert res["converged"] and res["rel_residual"] < 1e-10
+    bh = b.cpu().numpy()
+    rr = bh - orc.spmv(_o(A), x.cpu().numpy(), alg="parallel", nthreads=orc.max_threads())
+    assert np.linalg.norm(rr) <= 1.01e-10 * np.linalg.norm(bh)
