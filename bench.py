@@ -238,6 +238,17 @@ def bench_c3(la, torch, dev, stream, args):
     t0 = time.time()
     A = la.gen.powerlaw(4_000_000, 16.0, 1.0, 20000, seed=3)
     prep = round(time.time() - t0, 3)
+    if args.pmc_probe:  # 4 FP64 SpMVs + a 1 GiB copy, for rocprofv3 --pmc
+        d = la.DeviceMatrix(A, stream=stream)
+        x = torch.tensor(la.gen.uniform(A.n, 7), device=dev)
+        y = torch.zeros_like(x)
+        for _ in range(4):
+            d.spmv(x, y, alg=la.SPMV_ALBUS)
+        big = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+        torch.empty_like(big).copy_(big)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "c3 spmv x4 + copy 1GiB", "nnz": A.nnz}))
+        return
     rl = np.diff(A.ai)
     mats = [la.DeviceMatrix(A, stream=stream) for _ in range(2)]
     tab = _spmv_table(la, torch, A, mats, dev, stream,
