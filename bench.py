@@ -309,9 +309,13 @@ def bench_c4(la, torch, dev, stream, args):
                                                        inner_tol=1e-3)),
                 ("mixed_bf16", dict(precision=la.PREC_MIXED_BF16, inner_tol=1e-3)),
                 ("fp64_jacobi6", dict(precision=la.PREC_FP64, **J)),
-                ("mixed_f32_jacobi6", dict(precision=la.PREC_MIXED_F32, inner_tol=1e-5, **J)),
+                ("mixed_f32_jacobi6", dict(precision=la.PREC_MIXED_F32, inner_tol=1e-4, **J)),
                 ("mixed_bf16_matrix_f32_factors_jacobi6",
-                 dict(precision=la.PREC_MIXED_BF16_F32P, inner_tol=1e-3, **J))]
+                 dict(precision=la.PREC_MIXED_BF16_F32P, inner_tol=1e-3, **J)),
+                # beyond the configuration's FP32 factors: BF16 factors with
+                # the matrix kept in FP32
+                ("mixed_f32_matrix_bf16_factors_jacobi6",
+                 dict(precision=la.PREC_MIXED_F32_BF16P, inner_tol=1e-4, **J))]
     for name, kw in variants:
         kw = dict(kw)
         kw.setdefault("trsv_mode", mode)

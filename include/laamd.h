@@ -294,8 +294,10 @@ typedef enum {
   LA_PREC_FP64 = 0,       /* everything FP64                              */
   LA_PREC_MIXED_F32 = 1,  /* inner solve: FP32 matrix/factors/vectors      */
   LA_PREC_MIXED_BF16 = 2, /* inner solve: BF16 matrix+factors, FP32 vectors*/
-  LA_PREC_MIXED_BF16_F32P = 3 /* inner solve: BF16 matrix, FP32 factors and
-                                 vectors (no counterpart in the reference)   */
+  LA_PREC_MIXED_BF16_F32P = 3, /* inner solve: BF16 matrix, FP32 factors and
+                                  vectors (no counterpart in the reference)  */
+  LA_PREC_MIXED_F32_BF16P = 4   /* inner solve: FP32 matrix and vectors, BF16
+                                  factors (no counterpart in the reference)  */
 } la_precision;
 
 typedef enum {

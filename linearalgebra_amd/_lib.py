@@ -15,7 +15,7 @@ F64, F32, BF16 = 0, 1, 2
 SPMV_ROW, SPMV_SEGSUM, SPMV_ALBUS, SPMV_AUTO, SPMV_SELL = 0, 1, 2, 3, 4
 TRSV_LEVEL, TRSV_SYNCFREE, TRSV_JACOBI = 0, 1, 2
 LOWER, UPPER = 0, 1
-PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16, PREC_MIXED_BF16_F32P = 0, 1, 2, 3
+PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16, PREC_MIXED_BF16_F32P, PREC_MIXED_F32_BF16P = 0, 1, 2, 3, 4
 STOP_TRUE_RESIDUAL, STOP_RECURRENCE = 0, 1
 
 

@@ -73,7 +73,8 @@ la_dtype inner_val_dtype(int precision) {
                                                                                  : LA_F32;
 }
 la_dtype inner_prec_dtype(int precision) {
-  return precision == LA_PREC_MIXED_BF16 ? LA_BF16 : LA_F32;
+  return precision == LA_PREC_MIXED_BF16 || precision == LA_PREC_MIXED_F32_BF16P ? LA_BF16
+                                                                                 : LA_F32;
 }
 
 // z = M^-1 r, or M^-T r when the solve is with A^T
@@ -336,7 +337,7 @@ using clk = std::chrono::steady_clock;
 template <typename Core>
 int run_solver(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
                la_solver_result *res, Core core, bool is_pcg) {
-  LA_CHECK_ARG(o.precision >= LA_PREC_FP64 && o.precision <= LA_PREC_MIXED_BF16_F32P,
+  LA_CHECK_ARG(o.precision >= LA_PREC_FP64 && o.precision <= LA_PREC_MIXED_F32_BF16P,
                "unknown precision policy");
   LA_CHECK_ARG(!(o.reliable_delta >= 1.0), "reliable_delta must be below 1");
   hipStream_t s = A.stream;

@@ -47,7 +47,8 @@ def test_pcg_ic0_fp64(dev, mode, stop):
     assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-7 * np.linalg.norm(xo)
 
 
-@pytest.mark.parametrize("prec", [la.PREC_MIXED_F32, la.PREC_MIXED_BF16])
+@pytest.mark.parametrize("prec", [la.PREC_MIXED_F32, la.PREC_MIXED_BF16,
+                                  la.PREC_MIXED_F32_BF16P])
 def test_pcg_mixed_reaches_fp64_tolerance(dev, prec):
     import torch
     A, R = _ic_problem(rcm=True)
@@ -140,7 +141,7 @@ def test_fgmres_ilu(dev, mode):
 
 
 @pytest.mark.parametrize("prec", [la.PREC_MIXED_F32, la.PREC_MIXED_BF16,
-                                  la.PREC_MIXED_BF16_F32P])
+                                  la.PREC_MIXED_BF16_F32P, la.PREC_MIXED_F32_BF16P])
 def test_fgmres_mixed_ir(dev, prec):
     import torch
     A = _o(la.gen.aniso27(14, 14, 14, aniso=100.0))
