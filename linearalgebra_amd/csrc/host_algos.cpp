@@ -3,12 +3,15 @@
 // Everything here is serial, deterministic and bit-exact with the
 // reference's definitions (cited per function); none of it touches the GPU.
 #include "host_algos.hpp"
+#include "team.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <limits>
 #include <numeric>
+#include <stdexcept>
 
 namespace laamd {
 
@@ -353,9 +356,105 @@ void rcm_serial(int n, int base, const int *ai, const int *aj, int *iperm, int *
 // row lives in a sorted linked list over a dense column index (next[], HEAD
 // = n), which is the reference's forward_list laid out in arrays.
 // first_upper is produced for level 0 as well (see the numeric phase).
+namespace {
+// ILU(0) / ILU(1) symbolic with a team of threads, equal to the serial
+// rule below. A level-1 entry of lev_ij = min(lev_ij, lev_ik + lev_kj + 1)
+// can only come from lev_ik = lev_kj = 0, i.e. from entries of A: (i, j) is
+// in the ILU(1) pattern iff it is in A or some k < i has (i, k) in A and
+// (k, j) in A with j > k. Each row is then independent. Needs rows with
+// strictly increasing columns (the serial code's merge assumes it too).
+bool rows_strictly_sorted(team::Team &T, int n, int base, const int *ai, const int *aj) {
+  std::atomic<bool> ok{true};
+  T.run([&](int t, int k) {
+    int64_t b, e;
+    team::split(n, t, k, b, e);
+    for (int64_t i = b; i < e && ok.load(std::memory_order_relaxed); ++i)
+      for (int q = ai[i] - base + 1; q < ai[i + 1] - base; ++q)
+        if (aj[q] <= aj[q - 1]) {
+          ok.store(false, std::memory_order_relaxed);
+          break;
+        }
+  });
+  return ok.load();
+}
+
+void iluk_symbolic_le1(team::Team &T, int n, int base, const int *ai, const int *aj, int level,
+                       std::vector<int> &fai, std::vector<int> &faj,
+                       std::vector<int> &first_upper) {
+  const int nt = T.size();
+  std::vector<std::vector<int>> out(nt), len(nt);
+  T.run([&](int t, int k) {
+    int64_t rb, re;
+    team::split(n, t, k, rb, re);
+    auto &o = out[t];
+    auto &l = len[t];
+    l.assign(static_cast<size_t>(re - rb), 0);
+    std::vector<int> mark(level > 0 ? n : 0, -1), cand;
+    for (int64_t i = rb; i < re; ++i) {
+      const size_t start = o.size();
+      if (level == 0) {
+        for (int q = ai[i] - base; q < ai[i + 1] - base; ++q) o.push_back(aj[q]);
+      } else {
+        cand.clear();
+        for (int q = ai[i] - base; q < ai[i + 1] - base; ++q) {
+          const int c = aj[q] - base;
+          mark[c] = static_cast<int>(i);
+          cand.push_back(c);
+        }
+        for (int q = ai[i] - base; q < ai[i + 1] - base; ++q) {
+          const int m = aj[q] - base;
+          if (m >= i) break;  // sorted: the rest are not below i
+          for (int r = ai[m] - base; r < ai[m + 1] - base; ++r) {
+            const int c = aj[r] - base;
+            if (c <= m || mark[c] == i) continue;
+            mark[c] = static_cast<int>(i);
+            cand.push_back(c);
+          }
+        }
+        std::sort(cand.begin(), cand.end());
+        for (int c : cand) o.push_back(c + base);
+      }
+      l[i - rb] = static_cast<int>(o.size() - start);
+    }
+  });
+  fai.assign(static_cast<size_t>(n) + 1, base);
+  std::vector<int64_t> off(static_cast<size_t>(nt) + 1, 0);
+  for (int t = 0; t < nt; ++t) off[t + 1] = off[t] + static_cast<int64_t>(out[t].size());
+  if (off[nt] + base > INT_MAX) throw std::length_error("ILU pattern exceeds int32 indices");
+  faj.resize(static_cast<size_t>(off[nt]));
+  first_upper.assign(n, 0);
+  // row pointers: per member, then shifted by the members before it
+  T.run([&](int t, int k) {
+    int64_t rb, re;
+    team::split(n, t, k, rb, re);
+    int64_t p = off[t];
+    for (int64_t i = rb; i < re; ++i) {
+      const int64_t rs = p;
+      p += len[t][i - rb];
+      fai[i + 1] = static_cast<int>(p) + base;
+      int64_t fu = p;  // first column above i, else the row's end
+      for (int64_t q = rs; q < p; ++q)
+        if (out[t][q - off[t]] - base > i) {
+          fu = q;
+          break;
+        }
+      first_upper[i] = static_cast<int>(fu);
+    }
+    std::copy(out[t].begin(), out[t].end(), faj.begin() + off[t]);
+  });
+}
+}  // namespace
+
 void iluk_symbolic(int n, int base, const int *ai, const int *aj, int level,
                    std::vector<int> &fai, std::vector<int> &faj,
                    std::vector<int> &first_upper) {
+  if (level <= 1 && n >= 4096) {
+    team::Team T(host_threads());
+    if (rows_strictly_sorted(T, n, base, ai, aj)) {
+      iluk_symbolic_le1(T, n, base, ai, aj, level, fai, faj, first_upper);
+      return;
+    }
+  }
   fai.assign(static_cast<size_t>(n) + 1, base);
   first_upper.assign(n, 0);
   faj.clear();

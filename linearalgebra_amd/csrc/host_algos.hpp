@@ -29,7 +29,6 @@ void rcm(int n, int base, const int *ai, const int *aj, int *iperm, int *perm,
          int nthreads = 0);
 void rcm_serial(int n, int base, const int *ai, const int *aj, int *iperm, int *perm);
 bool pattern_symmetric(int n, int base, const int *ai, const int *aj);
-int rcm_default_threads();
 void iluk_symbolic(int n, int base, const int *ai, const int *aj, int level,
                    std::vector<int> &fai, std::vector<int> &faj,
                    std::vector<int> &first_upper);

@@ -248,3 +248,51 @@ def test_rcm_threads_bit_exact(name, make):
         i2, p2 = la.rcm(as_lib_csr(A), nthreads=nt)
         np.testing.assert_array_equal(i1, i2, err_msg="threads %d" % nt)
         np.testing.assert_array_equal(p1, p2, err_msg="threads %d" % nt)
+
+
+def _ilu_graphs():
+    """sorted-row patterns with n >= 4096 (the threaded ILU(0)/ILU(1)
+    symbolic), symmetric and unsymmetric, a missing diagonal, base 1"""
+    import scipy.sparse as sp
+
+    def csr(M, base=0):
+        M = sp.csr_matrix(M)
+        M.sum_duplicates()
+        M.sort_indices()
+        return orc.CSR(M.shape[0], (M.indptr + base).astype(np.int32),
+                       (M.indices + base).astype(np.int32), M.data.astype(float))
+
+    def aniso(g):
+        A = la.gen.aniso27(g)
+        return sp.csr_matrix((A.av, A.aj, A.ai), shape=(A.n, A.n))
+
+    def unsym(n, seed):
+        M = sp.random(n, n, density=6.0 / n, random_state=seed, format="csr")
+        return M + sp.eye(n)
+
+    def nodiag(n, seed):
+        M = unsym(n, seed).tolil()
+        for i in range(0, n, 7):
+            M[i, i] = 0.0
+        M = M.tocsr()
+        M.eliminate_zeros()
+        return M
+
+    return [("aniso27_18", lambda: csr(aniso(18))),
+            ("unsym_9000", lambda: csr(unsym(9000, 3))),
+            ("missing_diag_6000", lambda: csr(nodiag(6000, 5))),
+            ("aniso27_16_base1", lambda: csr(aniso(16), base=1))]
+
+
+@pytest.mark.parametrize("level", [0, 1])
+@pytest.mark.parametrize("name,make", _ilu_graphs(), ids=[g[0] for g in _ilu_graphs()])
+def test_iluk_symbolic_threaded_bit_exact(name, make, level):
+    """The threaded ILU(0)/ILU(1) symbolic phase (the two-hop form of the
+    level rule) equals the oracle's serial restatement of
+    incomplete_lu.cpp:153-275: pattern and first-upper positions."""
+    A = make()
+    ref_ai, ref_aj, ref_fu = orc.iluk_symbolic(A, level)
+    got_ai, got_aj, got_fu = la.iluk_symbolic(as_lib_csr(A), level)
+    np.testing.assert_array_equal(np.asarray(got_ai), np.asarray(ref_ai))
+    np.testing.assert_array_equal(np.asarray(got_aj), np.asarray(ref_aj))
+    np.testing.assert_array_equal(np.asarray(got_fu), np.asarray(ref_fu))
