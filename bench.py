@@ -27,7 +27,8 @@ int32 indices, RCM-reordered on the host.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4]
        [--no-pcg] [--no-cpu] [--pmc-probe] [--trsv level|syncfree] [--c4-grid G]
 
---config c3 (power-law SpMV in FP64/FP32/BF16), --config c4 (27-point
+--config c1 (the 256^2 Poisson chain next to the CPU oracle), --config c3
+(power-law SpMV in FP64/FP32/BF16), --config c4 (27-point
 160^3: SpMV, ILU(1)-GMRES(30) FP64 vs mixed precision) and --config c5
 (the adjoint of a 512^2 diffusion solve) print their own JSON records; the
 default (c2) prints the bench line.
@@ -230,6 +231,81 @@ def _spmv_table(la, torch, A, mats, dev, stream, algs, dtypes, steps, seed=77):
                                             "rel_err_vs_fp64": float("%.3e" % err)}
             log(aname, name, out["%s_%s" % (aname, name)])
     return out
+
+
+def bench_c1(la, torch, dev, stream, args):
+    """The reference's CPU-runnable chain: 2-D 5-point Poisson 256^2
+    (n = 65,536), SpMV y = A x (x ~ U[-1,1), seeded), ILU(0) factorisation
+    and its forward/backward solve, IC(0)-PCG on b = A 1 to a 1e-8 true
+    relative residual; each step on the GPU (device-resident, FP64) next to
+    the oracle's serial restatement on one host core, with the GPU result's
+    relative difference from the oracle's."""
+    from oracle import oracle as orc
+    A = la.gen.poisson2d(256)
+    Ao = orc.CSR(A.n, A.ai, A.aj, A.av)
+    xh = la.gen.uniform(A.n, 101)
+    bh = orc.spmv(Ao, np.ones(A.n))
+    out = {"config": "c1 poisson2d 256^2", "n": A.n, "nnz": A.nnz}
+
+    def gpu_time(fn, reps=20):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    def cpu_time(fn, reps=5):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        return (time.perf_counter() - t0) / reps, r
+    d = la.DeviceMatrix(A, stream=stream)
+    x, y = torch.tensor(xh, device=dev), torch.zeros(A.n, dtype=torch.float64, device=dev)
+    t_gpu = gpu_time(lambda: d.spmv(x, y))
+    t_cpu, yr = cpu_time(lambda: orc.spmv(Ao, xh))
+    out["spmv"] = {"gpu_us": round(t_gpu * 1e6, 2), "cpu_us": round(t_cpu * 1e6, 1),
+                   "rel_diff": float("%.2e" % (np.linalg.norm(y.cpu().numpy() - yr) /
+                                               np.linalg.norm(yr)))}
+    t0 = time.perf_counter()
+    F = la.iluk(A, 0)
+    t_fact = time.perf_counter() - t0
+    P = la.Preconditioner(F, "ilu")
+    P.set_stream(stream)
+    b = torch.tensor(bh, device=dev)
+    z = torch.zeros_like(b)
+    Fo = orc.CSR(F.n, F.ai, F.aj, F.av)
+    t_gpu = gpu_time(lambda: P.apply(b, z))
+    t_cpu, zr = cpu_time(lambda: orc.prec_apply(Ao, "ilu", Fo, bh))
+    out["ilu0"] = {"host_factor_s": round(t_fact, 4), "levels_L_U": P.levels(),
+                   "apply_gpu_us": round(t_gpu * 1e6, 1), "apply_cpu_us": round(t_cpu * 1e6, 1),
+                   "rel_diff": float("%.2e" % (np.linalg.norm(z.cpu().numpy() - zr) /
+                                               np.linalg.norm(zr)))}
+    R, ok = la.icc(la.split_triangle(A, upper=True), 0)
+    Pc = la.Preconditioner(R, "ic")
+    Pc.set_stream(stream)
+    Ro = orc.CSR(R.n, R.ai, R.aj, R.av)
+    runs = {}
+    for name, kw in [("fp64", {}), ("mixed_bf16_jacobi3",
+                                    dict(precision=la.PREC_MIXED_BF16,
+                                         trsv_mode=la.TRSV_JACOBI))]:
+        xs = torch.zeros_like(b)
+        la.pcg(d, Pc, b, xs, rel_tol=1e-8, max_iters=2000, **kw)
+        xs.zero_()
+        torch.cuda.synchronize()
+        r = la.pcg(d, Pc, b, xs, rel_tol=1e-8, max_iters=2000, **kw)
+        runs[name] = {"seconds": round(r["seconds"], 5), "iters": r["iters"],
+                      "rel_residual": float("%.2e" % r["rel_residual"]),
+                      "x_minus_ones": float("%.2e" % (np.linalg.norm(xs.cpu().numpy() - 1.0) /
+                                                       np.sqrt(A.n)))}
+    t0 = time.perf_counter()
+    xo, conv, it, rr = orc.pcg(Ao, bh, kind="ic", F=Ro, rel_tol=1e-8, maxiter=2000)
+    runs["cpu_oracle_fp64_serial"] = {"seconds": round(time.perf_counter() - t0, 4), "iters": it,
+                                      "rel_residual": float("%.2e" % rr)}
+    out["ic0_pcg"] = runs
+    print(json.dumps(out))
 
 
 def bench_c3(la, torch, dev, stream, args):
@@ -470,6 +546,8 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
 
+    if args.config == "c1":
+        return bench_c1(la, torch, dev, stream, args)
     if args.config == "c3":
         return bench_c3(la, torch, dev, stream, args)
     if args.config == "c4":
@@ -477,7 +555,7 @@ def main():
     if args.config == "c5":
         return bench_c5(la, torch, dev, stream, args)
     if args.config != "c2":
-        raise SystemExit("--config is one of c2 (the bench line), c3, c4, c5")
+        raise SystemExit("--config is one of c2 (the bench line), c1, c3, c4, c5")
     A, prep = build_c2(la)
     n, nnz = A.n, A.nnz
     log("matrix n=%d nnz=%d prep=%s" % (n, nnz, prep))
