@@ -165,11 +165,12 @@ def run_pcg(la, torch, A, dev, stream, trsv_mode, variants, cpu=False):
            "analysis_s": round(t_levels, 3),
            "device_factor": {"setup_s": round(t_gsetup, 3), "numeric_s": round(t_gnum, 5),
                              "bitwise_equal_host": same},
-           "trsv_mode": ["level", "syncfree"][trsv_mode], "jacobi_sweeps": 3, "runs": {}}
+           "trsv_mode": ["level", "syncfree"][trsv_mode], "runs": {}}
     xs_all = {}
     for name, kw in variants:
         kw = dict(kw)
         kw.setdefault("trsv_mode", trsv_mode)
+        P.set_sweeps(kw.pop("sweeps", 3))
         x = torch.zeros(A.n, dtype=torch.float64, device=dev)
         # one untimed run per variant so level graphs and precision copies
         # exist; then the timed one
@@ -633,34 +634,36 @@ def main():
     if not args.no_pcg:
         del mats[1:]
         mode = la.TRSV_LEVEL if args.trsv == "level" else la.TRSV_SYNCFREE
-        J = dict(trsv_mode=la.TRSV_JACOBI)
         variants = [("fp64", dict(precision=la.PREC_FP64, stop_rule=la.STOP_TRUE_RESIDUAL)),
                     ("fp64_recurrence", dict(precision=la.PREC_FP64,
                                              stop_rule=la.STOP_RECURRENCE)),
                     ("mixed_f32", dict(precision=la.PREC_MIXED_F32)),
-                    ("mixed_bf16", dict(precision=la.PREC_MIXED_BF16)),
-                    ("fp64_jacobi3", dict(precision=la.PREC_FP64,
-                                          stop_rule=la.STOP_TRUE_RESIDUAL, **J)),
-                    ("fp64_jacobi3_recurrence", dict(precision=la.PREC_FP64,
-                                                     stop_rule=la.STOP_RECURRENCE, **J)),
-                    ("mixed_f32_jacobi3", dict(precision=la.PREC_MIXED_F32, **J)),
-                    ("mixed_bf16_jacobi3", dict(precision=la.PREC_MIXED_BF16, **J))]
+                    ("mixed_bf16", dict(precision=la.PREC_MIXED_BF16))]
+        for k in (3, 1):  # Jacobi sweeps per triangle
+            J = dict(trsv_mode=la.TRSV_JACOBI, sweeps=k)
+            variants += [("fp64_jacobi%d" % k, dict(precision=la.PREC_FP64,
+                                                    stop_rule=la.STOP_TRUE_RESIDUAL, **J)),
+                         ("fp64_jacobi%d_recurrence" % k, dict(precision=la.PREC_FP64,
+                                                               stop_rule=la.STOP_RECURRENCE,
+                                                               **J)),
+                         ("mixed_f32_jacobi%d" % k, dict(precision=la.PREC_MIXED_F32, **J)),
+                         ("mixed_bf16_jacobi%d" % k, dict(precision=la.PREC_MIXED_BF16, **J))]
         pcg = run_pcg(la, torch, A, dev, stream, mode, variants, cpu=not args.no_cpu)
         r = pcg["runs"]
         sec = {k: v["seconds"] for k, v in r.items()}
-        best_mixed = min(("mixed_f32", "mixed_bf16", "mixed_f32_jacobi3", "mixed_bf16_jacobi3"),
-                         key=lambda k: sec[k])
+        mixed = [k for k in sec if k.startswith("mixed")]
+        pcg["best_mixed"] = min(mixed, key=lambda k: sec[k])
         # against the plain FP64 solve (the reference's algorithm) ...
-        pcg["speedup_vs_plain_fp64"] = {k: round(sec["fp64"] / sec[k], 3)
-                                        for k in sec if k.startswith("mixed")}
+        pcg["speedup_vs_plain_fp64"] = {k: round(sec["fp64"] / sec[k], 3) for k in mixed}
+
         # ... and against the fastest FP64 solve with the same triangular
-        # solves (either stop rule)
-        ex64 = min(sec["fp64"], sec["fp64_recurrence"])
-        jac64 = min(sec["fp64_jacobi3"], sec["fp64_jacobi3_recurrence"])
-        pcg["speedup_vs_fp64_same_trsv"] = {
-            k: round((jac64 if "jacobi" in k else ex64) / sec[k], 3)
-            for k in sec if k.startswith("mixed")}
-        pcg["best_mixed"] = best_mixed
+        # solves (same sweep count; either stop rule)
+        def same64(k):
+            tag = k.split("_", 2)[2] if k.count("_") >= 2 else ""
+            if not tag:
+                return min(sec["fp64"], sec["fp64_recurrence"])
+            return min(sec["fp64_" + tag], sec["fp64_%s_recurrence" % tag])
+        pcg["speedup_vs_fp64_same_trsv"] = {k: round(same64(k) / sec[k], 3) for k in mixed}
 
     cpu = None
     if not args.no_cpu:
