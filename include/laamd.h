@@ -136,6 +136,17 @@ int la_icc_numeric(int n, int base, const int *ai, const int *aj,
                    const double *av, const int *diag_pos, double omega,
                    const int *icc_ai, const int *icc_aj, double *icc_av,
                    int *ok);
+/* mkl_wrapper/incomplete_cholesky.cpp:195-274 incomplete_cholesky_k::
+ * numeric_factorize, one pass: same pattern and arguments as la_icc_numeric,
+ * but the diagonal gets the ADDITIVE shift, rows are visited in the
+ * reference's first-in first-out column order, and *ok = 0 on a pivot <= 0.
+ * The caller runs the retry loop: shift_0 = 1e-3 - min_i a_ii when
+ * min_i a_ii <= 0, else 0; then shift = max(1e-3, 2 shift), at most 20
+ * retries. */
+int la_ick_numeric(int n, int base, const int *ai, const int *aj,
+                   const double *av, const int *diag_pos, double shift,
+                   const int *icc_ai, const int *icc_aj, double *icc_av,
+                   int *ok);
 
 /* ====================== seeded synthetic inputs (host) ================= */
 /* Two-phase where the nnz is not closed-form: pass aj == NULL to get ai and

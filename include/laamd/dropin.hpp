@@ -22,9 +22,11 @@
 // failed.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -515,7 +517,7 @@ class device_preconditioner : public mkl_sparse_mat {
   la_prec_t prec_handle() const override { return P_.get(); }
   void set_trsv_mode(la_trsv_mode m) { trsv_mode_ = m; }
   la_trsv_mode trsv_mode() const { return trsv_mode_; }
-  // sweeps per triangle of LA_TRSV_JACOBI applies (an extension of this build)
+  // sweeps per triangle of LA_TRSV_JACOBI applies (no counterpart in the reference)
   void set_sweeps(int k) { detail::check(la_prec_set_sweeps(P_.get(), k)); }
 
  protected:
@@ -568,11 +570,11 @@ class incomplete_lu_k : public device_preconditioner {
   std::vector<int> first_upper_;
 };
 
-// incomplete_cholesky_k_2 (incomplete_cholesky.cpp:286-342): IC(k) symbolic
-// of the upper triangle of A, numeric with a multiplicative diagonal shift
-// doubled from 1e-3 up to 20 times when shift(true). incomplete_cholesky_k
-// (the additive-shift variant, :74-281) maps to the same factorisation.
-class incomplete_cholesky_k_2 : public device_preconditioner {
+// Symbolic phase shared by both IC(k) classes: the level-of-fill pattern of
+// the upper triangle of A (incomplete_cholesky.cpp:286-313 via
+// ICCLevelSymbolic; incomplete_cholesky_k's own symbolic phase at :74-193
+// yields the same pattern, tests/precond_test.cpp:63-116).
+class incomplete_cholesky_common : public device_preconditioner {
  public:
   void set_level(int level) { _level = level; }
   void shift(bool s) { _shift = s; }
@@ -599,6 +601,20 @@ class incomplete_cholesky_k_2 : public device_preconditioner {
     _nnz = static_cast<int>(nnz);
     return true;
   }
+
+ protected:
+  int _level = 0;
+  bool _shift = false;
+  double _initial_shift = 1e-3;
+  int _nrestart = 20;
+  std::vector<int> diag_pos_;
+};
+
+// incomplete_cholesky_k_2 (incomplete_cholesky.cpp:286-342): numeric phase
+// with a multiplicative diagonal shift a_ii (1 + s), s doubled from 1e-3 up
+// to 20 times when shift(true); fails on a negative pivot.
+class incomplete_cholesky_k_2 : public incomplete_cholesky_common {
+ public:
   bool numeric_factorize(mkl_sparse_mat const *const A) {
     double shift = 0.0;
     for (int iter = 0;; ) {
@@ -614,15 +630,37 @@ class incomplete_cholesky_k_2 : public device_preconditioner {
     upload_factor(true);
     return true;
   }
-
- private:
-  int _level = 0;
-  bool _shift = false;
-  double _initial_shift = 1e-3;
-  int _nrestart = 20;
-  std::vector<int> diag_pos_;
 };
-using incomplete_cholesky_k = incomplete_cholesky_k_2;
+
+// incomplete_cholesky_k (incomplete_cholesky.cpp:195-274): numeric phase
+// with an ADDITIVE diagonal shift a_ii + s, fails on a pivot <= 0. With
+// shift(true) the first s is 1e-3 - min a_ii when min a_ii <= 0 (else 0),
+// then s = max(1e-3, 2 s) on each failure, at most 20 retries.
+class incomplete_cholesky_k : public incomplete_cholesky_common {
+ public:
+  bool numeric_factorize(mkl_sparse_mat const *const A) {
+    const double *av = A->get_av().get();
+    double shift = 0.0;
+    if (_shift) {
+      double min_diag = std::numeric_limits<double>::max();
+      for (int i = 0; i < _nrow; ++i) min_diag = std::min(min_diag, av[diag_pos_[i] - _base]);
+      if (min_diag <= 0.0) shift = _initial_shift - min_diag;
+      std::printf("init shift: %g\n", shift);
+    }
+    for (int iter = 0;; ) {
+      int ok = 0;
+      detail::check(la_ick_numeric(_nrow, _base, A->get_ai().get(), A->get_aj().get(), av,
+                                   diag_pos_.data(), shift, _ai.get(), _aj.get(), _av.get(),
+                                   &ok));
+      if (ok) break;
+      if (!_shift || ++iter > _nrestart) return false;
+      shift = std::max(_initial_shift, 2.0 * shift);
+      std::printf("shift: %g\n", shift);
+    }
+    upload_factor(true);
+    return true;
+  }
+};
 
 // mkl_wrapper/mkl_solver.h:10-87
 class mkl_solver {

@@ -24,7 +24,8 @@ __all__ = [
     "CSR", "DeviceMatrix", "TriangularSolve", "Preconditioner", "pcg",
     "fgmres", "topo_sort2", "split_ldu", "split_triangle", "diagonal",
     "transpose", "permute", "rcm", "iluk_symbolic", "iluk_numeric",
-    "icc_symbolic", "icc_numeric", "gen", "F64", "F32", "BF16", "LaError",
+    "icc_symbolic", "icc_numeric", "ick_numeric", "ick_shifted", "gen", "F64", "F32", "BF16",
+    "LaError",
     "read_matrix_market", "read_binary_coo", "read_binary_vector", "rowwise_sqrt_norm", "dtad",
 ]
 
@@ -552,6 +553,42 @@ def icc_shifted(U, lvl, initial_shift=1e-3, nrestart=20):
         if it > nrestart:
             raise LaError(_lib.LA_ERR_NUMERIC, "IC(k) failed after %d shifts" % nrestart)
         shift = max(2 * shift, initial_shift)
+
+
+def ick_numeric(U, diag_pos, ia, ja, shift=0.0):
+    """One pass of incomplete_cholesky_k::numeric_factorize
+    (mkl_wrapper/incomplete_cholesky.cpp:195-274): additive diagonal shift,
+    first-in first-out row order, fails on a pivot <= 0 (la_ick_numeric)."""
+    va = np.zeros(len(ja))
+    ok = C.c_int()
+    check(lib().la_ick_numeric(U.n, U.base, _p(U.ai), _p(U.aj), _p(U.av), _p(_i32(diag_pos)),
+                               float(shift), _p(_i32(ia)), _p(_i32(ja)), _p(va), C.byref(ok)))
+    return CSR(U.n, ia, ja, va), bool(ok.value)
+
+
+def ick_shifted(U, lvl, shift=True, initial_shift=1e-3, nrestart=20):
+    """incomplete_cholesky_k symbolic + numeric with its retry loop
+    (mkl_wrapper/incomplete_cholesky.cpp:195-274) on the upper triangle U
+    (diagonal first in each row) -> (R, shift used). With shift=True the
+    first shift is initial_shift - min a_ii when min a_ii <= 0 (else 0), and
+    each failure sets shift = max(initial_shift, 2 shift), at most nrestart
+    times; with shift=False a pivot <= 0 raises at once."""
+    dp = U.ai[:-1]
+    ia, ja = icc_symbolic(U, dp, lvl)
+    s = 0.0
+    if shift and U.n:
+        min_diag = float(np.min(np.asarray(U.av)[np.asarray(dp) - U.base]))
+        if min_diag <= 0:
+            s = initial_shift - min_diag
+    it = 0
+    while True:
+        R, ok = ick_numeric(U, dp, ia, ja, s)
+        if ok:
+            return R, s
+        it += 1
+        if not shift or it > nrestart:
+            raise LaError(_lib.LA_ERR_NUMERIC, "incomplete_cholesky_k: non-positive pivot")
+        s = max(initial_shift, 2 * s)
 
 
 # ---------------------------------------------------------------- generators

@@ -65,6 +65,7 @@ _SIGS = {
     "la_iluk_numeric": [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "la_icc_symbolic": [_i, _i, _vp, _vp, _vp, _i, _vp, _vp, C.POINTER(_i64)],
     "la_icc_numeric": [_i, _i, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp, C.POINTER(_i)],
+    "la_ick_numeric": [_i, _i, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp, C.POINTER(_i)],
     "la_gen_poisson2d": [_i, _i, _vp, _vp, _vp, C.POINTER(_i64)],
     "la_gen_laplace3d": [_i, _i, _i, _vp, _vp, _vp, C.POINTER(_i64)],
     "la_gen_powerlaw": [_i, _d, _d, _i, _u64, _vp, _vp, _vp, C.POINTER(_i64)],

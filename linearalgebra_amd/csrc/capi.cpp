@@ -225,6 +225,16 @@ int la_icc_numeric(int n, int base, const int *ai, const int *aj, const double *
   LA_CATCH
 }
 
+int la_ick_numeric(int n, int base, const int *ai, const int *aj, const double *av,
+                   const int *diag_pos, double shift, const int *icc_ai, const int *icc_aj,
+                   double *icc_av, int *ok) {
+  LA_TRY
+  LA_CHECK_ARG(n >= 0 && ai && diag_pos && icc_ai && icc_av, "null argument");
+  const bool good = ick_numeric(n, base, ai, aj, av, diag_pos, shift, icc_ai, icc_aj, icc_av);
+  if (ok) *ok = good ? 1 : 0;
+  LA_CATCH
+}
+
 // ------------------------------------------------------------ generators
 int la_gen_poisson2d(int nx, int ny, int *ai, int *aj, double *av, int64_t *nnz) {
   LA_TRY

@@ -660,4 +660,62 @@ bool icc_numeric(int n, int base, const int *ai, const int *aj, const double *av
   return true;
 }
 
+// mkl_wrapper/incomplete_cholesky.cpp:195-274 (incomplete_cholesky_k::
+// numeric_factorize, one pass): the same up-looking IC as icc_numeric but the
+// diagonal gets an ADDITIVE shift, the rows k with an entry in column j are
+// visited first-in first-out (the reference keeps one append-only vector per
+// column; here a per-column queue threaded through next[]), and a pivot <= 0
+// fails. The two visit orders round differently, so the factors agree only
+// to a few ulps. diag_pos is base-adjusted like icc_numeric's.
+bool ick_numeric(int n, int base, const int *ai, const int *aj, const double *av,
+                 const int *diag_pos, double shift, const int *icc_ai,
+                 const int *icc_aj, double *icc_av) {
+  const int NONE = -1;
+  std::vector<int> head(n, NONE), tail(n, NONE), next(n, NONE);
+  std::vector<int64_t> pos(n, 0);
+  auto enqueue = [&](int k, int c) {
+    next[k] = NONE;
+    if (head[c] == NONE) head[c] = k;
+    else next[tail[c]] = k;
+    tail[c] = k;
+  };
+  for (int j = 0; j < n; ++j) {
+    int64_t a = diag_pos[j] - base;
+    const int64_t ae = ai[j + 1] - base;
+    const int64_t ps = icc_ai[j] - base, pe = icc_ai[j + 1] - base;
+    for (int64_t p = ps; p < pe; ++p) {
+      if (a == ae || icc_aj[p] != aj[a]) icc_av[p] = 0.0;
+      else icc_av[p] = av[a++];
+    }
+    icc_av[ps] += shift;
+    for (int k = head[j]; k != NONE;) {
+      const int knext = next[k];
+      int64_t q = pos[k];
+      const int64_t qe = icc_ai[k + 1] - base;
+      if (q + 1 < qe) {
+        pos[k] = q + 1;
+        enqueue(k, icc_aj[q + 1] - base);
+      }
+      k = knext;
+      const double ukj = icc_av[q];
+      int64_t t = ps;
+      while (q < qe && t < pe) {
+        if (icc_aj[q] == icc_aj[t]) icc_av[t++] -= ukj * icc_av[q++];
+        else if (icc_aj[q] < icc_aj[t]) ++q;
+        else ++t;
+      }
+    }
+    head[j] = NONE;
+    if (pe - ps > 1) {
+      pos[j] = ps + 1;
+      enqueue(j, icc_aj[ps + 1] - base);
+    }
+    if (icc_av[ps] <= 0) return false;
+    const double d = std::sqrt(icc_av[ps]);
+    icc_av[ps] = d;
+    for (int64_t t = ps + 1; t < pe; ++t) icc_av[t] /= d;
+  }
+  return true;
+}
+
 }  // namespace laamd

@@ -41,6 +41,9 @@ void icc_symbolic(int n, int base, const int *ai, const int *aj,
 bool icc_numeric(int n, int base, const int *ai, const int *aj, const double *av,
                  const int *diag_pos, double omega, const int *icc_ai,
                  const int *icc_aj, double *icc_av);
+bool ick_numeric(int n, int base, const int *ai, const int *aj, const double *av,
+                 const int *diag_pos, double shift, const int *icc_ai,
+                 const int *icc_aj, double *icc_av);
 
 // seeded generators (gen.cpp)
 struct SplitMix64 {

@@ -715,6 +715,68 @@ int orc_icc_numeric(int n, int base, const int *ai, const int *aj,
   return ok;
 }
 
+/* mkl_wrapper/incomplete_cholesky.cpp:195-274, one pass of the do-while:
+ * column j's bucket lists the rows k < j whose next factor entry lies in
+ * column j, in the order they were appended (utils/utils.h:336-368 keeps one
+ * vector per column). Visiting bucket j appends each k to the bucket of its
+ * following column before k updates row j; row j joins the bucket of its
+ * first off-diagonal column once its bucket is done. Additive shift on the
+ * diagonal; a pivot <= 0 fails. */
+typedef struct { int k; long pos; } orc_kpos;
+
+int orc_ick_numeric(int n, int base, const int *ai, const int *aj,
+                    const double *av, const int *diag_pos, double shift,
+                    const int *icc_ai, const int *icc_aj, double *icc_av) {
+  int nn = n > 0 ? n : 1;
+  orc_kpos **bucket = (orc_kpos **)calloc(nn, sizeof(orc_kpos *));
+  int *cnt = (int *)calloc(nn, sizeof(int));
+  int *cap = (int *)calloc(nn, sizeof(int));
+  int ok = 1;
+#define ORC_PUSH(c, kk, pp)                                                   \
+  do {                                                                        \
+    int c_ = (c);                                                             \
+    if (cnt[c_] == cap[c_]) {                                                 \
+      cap[c_] = cap[c_] ? 2 * cap[c_] : 4;                                    \
+      bucket[c_] = (orc_kpos *)realloc(bucket[c_], sizeof(orc_kpos) * cap[c_]); \
+    }                                                                         \
+    bucket[c_][cnt[c_]].k = (kk);                                             \
+    bucket[c_][cnt[c_]++].pos = (pp);                                         \
+  } while (0)
+  for (int j = 0; j < n && ok; j++) {
+    long a = diag_pos[j] - base, a_end = ai[j + 1] - base;
+    long ps = icc_ai[j] - base, pe = icc_ai[j + 1] - base;
+    for (long p = ps; p < pe; p++) {
+      if (a == a_end || icc_aj[p] != aj[a]) icc_av[p] = 0;
+      else icc_av[p] = av[a++];
+    }
+    icc_av[ps] += shift;
+    for (int e = 0; e < cnt[j]; e++) {
+      int k = bucket[j][e].k;
+      long ki = bucket[j][e].pos, kend = icc_ai[k + 1] - base;
+      if (ki + 1 != kend) ORC_PUSH(icc_aj[ki + 1] - base, k, ki + 1);
+      const double akj = icc_av[ki];
+      long q = ps;
+      while (ki != kend && q != pe) {
+        if (icc_aj[q] == icc_aj[ki]) icc_av[q++] -= akj * icc_av[ki++];
+        else if (icc_aj[q] < icc_aj[ki]) q++;
+        else ki++;
+      }
+    }
+    free(bucket[j]);
+    bucket[j] = NULL;
+    cnt[j] = cap[j] = 0;
+    if (pe - ps != 1) ORC_PUSH(icc_aj[ps + 1] - base, j, ps + 1);
+    if (icc_av[ps] <= 0) { ok = 0; break; }
+    const double d = sqrt(icc_av[ps]);
+    icc_av[ps] = d;
+    for (long q = ps + 1; q < pe; q++) icc_av[q] /= d;
+  }
+#undef ORC_PUSH
+  for (int c = 0; c < nn; c++) free(bucket[c]);
+  free(bucket); free(cnt); free(cap);
+  return ok;
+}
+
 /* ------------------------------------------------------------- Krylov */
 
 static double dot(int n, const double *a, const double *b) {

@@ -130,6 +130,19 @@ int orc_icc_numeric(int n, int base, const int *ai, const int *aj,
                     const double *av, const int *diag_pos, double omega,
                     const int *icc_ai, const int *icc_aj, double *icc_av);
 
+/* incomplete_cholesky_k::numeric_factorize, one pass
+ * (mkl_wrapper/incomplete_cholesky.cpp:195-274): same pattern and diag_pos
+ * as orc_icc_numeric, ADDITIVE diagonal shift, rows visited first-in
+ * first-out per column, fails (returns 0) on a pivot <= 0. pinned (the
+ * textbook IC(0) recurrence of the 5-point grid in the reference's visit
+ * order, bitwise; tridiagonal -> Cholesky of A + shift I; (R^T R)_ij =
+ * a_ij + shift d_ij on the IC(0) pattern; a zero pivot fails here and
+ * passes orc_icc_numeric; within rounding of orc_icc_numeric, as
+ * tests/precond_test.cpp:118-164 prints). */
+int orc_ick_numeric(int n, int base, const int *ai, const int *aj,
+                    const double *av, const int *diag_pos, double shift,
+                    const int *icc_ai, const int *icc_aj, double *icc_av);
+
 /* ---- preconditioned Krylov (mkl_wrapper/mkl_solver.cpp) ---- */
 /* prec_kind: 0 none, 1 ILU (combined LU factor CSR, apply trsv(L unit) then
  * trsv(U non-unit), incomplete_lu.cpp:36-48), 2 IC (upper factor incl. diag,

@@ -72,6 +72,8 @@ def _declare(L):
     L.orc_icc_symbolic.restype = i
     L.orc_icc_numeric.argtypes = [i, i, _I, _I, _D, _I, d, _I, _I, _D]
     L.orc_icc_numeric.restype = i
+    L.orc_ick_numeric.argtypes = [i, i, _I, _I, _D, _I, d, _I, _I, _D]
+    L.orc_ick_numeric.restype = i
     L.orc_pcg.argtypes = [i, i, _I, _I, _D, C.POINTER(_Prec), _D, _D, d, d, i,
                           C.POINTER(i), C.POINTER(d)]
     L.orc_pcg.restype = i
@@ -279,6 +281,17 @@ def icc(U, lvl, omega=0.0, diag_pos=None):
     ia, ja = icc_symbolic(U, dp, lvl)
     va = np.zeros(len(ja))
     ok = lib().orc_icc_numeric(U.n, U.base, U.ai, U.aj, U.av, _i32(dp), omega,
+                               ia, ja, va)
+    return CSR(U.n, ia, ja, va), bool(ok)
+
+
+def ick(U, lvl, shift=0.0, diag_pos=None):
+    """One pass of incomplete_cholesky_k's numeric phase on the IC(lvl)
+    pattern of the upper triangle U: (R, ok), additive diagonal shift."""
+    dp = U.ai[:-1] if diag_pos is None else diag_pos
+    ia, ja = icc_symbolic(U, dp, lvl)
+    va = np.zeros(len(ja))
+    ok = lib().orc_ick_numeric(U.n, U.base, U.ai, U.aj, U.av, _i32(dp), shift,
                                ia, ja, va)
     return CSR(U.n, ia, ja, va), bool(ok)
 

@@ -4,6 +4,7 @@
 // IC(0)/ILU(k) preconditioners and the PCG/FGMRES solvers on a 2-D Poisson
 // matrix, against plain serial loops. Prints PASS/FAIL per check; exit code
 // is the number of failures.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -107,9 +108,9 @@ int main() {
   ok = pcg.solve(rhs.data(), sol.data());
   expect(ok && rel(sol, b) < 1e-7, "mkl_pcg_solver + IC(0)", rel(sol, b));
   expect(pcg.last_result().rel_residual < 1e-10, "PCG true residual", pcg.last_result().rel_residual);
-  // this build's extensions on the same solver: Jacobi-sweep triangular
-  // solves and mixed BF16 precision with reliable updates, still to an FP64
-  // true residual of 1e-10
+  // extensions with no counterpart in the reference, on the same solver:
+  // Jacobi-sweep triangular solves and mixed BF16 precision with reliable
+  // updates, still to an FP64 true residual of 1e-10
   prec->set_trsv_mode(LA_TRSV_JACOBI);
   prec->set_sweeps(3);
   pcg.set_precision(LA_PREC_MIXED_BF16);
@@ -122,6 +123,39 @@ int main() {
          pcg.last_result().outer_iters);
   pcg.set_precision(LA_PREC_FP64);
   prec->set_trsv_mode(LA_TRSV_SYNCFREE);
+
+  // incomplete_cholesky_k_2 factors the same IC(0) pattern with the rows of
+  // each column visited in the opposite order: the factors agree to rounding
+  {
+    auto p2 = std::make_shared<mkl_wrapper::incomplete_cholesky_k_2>();
+    p2->set_level(0);
+    ok = p2->symbolic_factorize(&Asym) && p2->numeric_factorize(&Asym);
+    double dmax = ok && p2->nnz() == prec->nnz() ? 0.0 : 1.0;
+    for (int q = 0; ok && q < prec->nnz(); ++q) {
+      const double a = prec->get_av()[q], c = p2->get_av()[q];
+      dmax = std::max(dmax, std::fabs(a - c) / std::fabs(c));
+    }
+    expect(dmax < 1e-13, "incomplete_cholesky_k_2 vs incomplete_cholesky_k", dmax);
+  }
+  // additive shift (incomplete_cholesky.cpp:207-266): [[-1, .5], [.5, 2]]
+  // starts at s = 1e-3 + 1, fails on the second pivot, retries at 2.002;
+  // r_11 = sqrt(-1 + 2.002)
+  {
+    mkl_wrapper::mkl_sparse_mat S2(2, 2, std::vector<int>{0, 2, 4}, std::vector<int>{0, 1, 0, 1},
+                                   std::vector<double>{-1.0, 0.5, 0.5, 2.0});
+    mkl_wrapper::mkl_sparse_mat_sym S2sym(S2);
+    mkl_wrapper::incomplete_cholesky_k p3;
+    p3.set_level(0);
+    p3.shift(true);
+    ok = p3.symbolic_factorize(&S2sym) && p3.numeric_factorize(&S2sym);
+    const double want = std::sqrt(-1.0 + 2 * (1e-3 + 1.0));
+    expect(ok && p3.get_av()[0] == want, "incomplete_cholesky_k additive shift retry",
+           ok ? p3.get_av()[0] - want : 1.0);
+    mkl_wrapper::incomplete_cholesky_k p4;
+    p4.set_level(0);
+    ok = p4.symbolic_factorize(&S2sym) && p4.numeric_factorize(&S2sym);
+    expect(!ok, "incomplete_cholesky_k without shift rejects a pivot <= 0", 0);
+  }
 
   // ILU(1)-FGMRES through the factory, the "gmres+iluk" registration of
   // execs/test1.cpp:61-77

@@ -47,6 +47,31 @@ def test_pcg_ic0_fp64(dev, mode, stop):
     assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-7 * np.linalg.norm(xo)
 
 
+def test_pcg_ic_k_additive(dev):
+    """PCG with the incomplete_cholesky_k factor (additive shift,
+    incomplete_cholesky.cpp:195-274) built by the product's host phase on
+    the RCM-ordered 24^3 Laplacian; the oracle's PCG with the oracle's
+    factor of the same matrix is the reference solution."""
+    import torch
+    A = _o(la.gen.laplace3d(24, 24, 24))
+    ip, p = orc.rcm(A)
+    A = orc.permute(A, ip, p)
+    U = orc.split_triangle(A, True)
+    R, s = la.ick_shifted(as_lib_csr(U), 1)
+    Ro, ok = orc.ick(U, 1, shift=s)
+    assert ok
+    b = la.gen.uniform(A.n, 23)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    P = la.Preconditioner(R, "ic")
+    x = torch.zeros(A.n, dtype=torch.float64, device=dev)
+    res = la.pcg(dA, P, torch.tensor(b, device=dev), x, rel_tol=1e-10, max_iters=2000)
+    print(res, flush=True)
+    assert res["converged"] and res["rel_residual"] < 1e-10
+    xo, ok, it, rr = orc.pcg(A, b, kind="ic", F=Ro, rel_tol=1e-10, maxiter=2000)
+    assert ok and abs(res["iters"] - it) <= 2
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-7 * np.linalg.norm(xo)
+
+
 @pytest.mark.parametrize("prec", [la.PREC_MIXED_F32, la.PREC_MIXED_BF16,
                                   la.PREC_MIXED_F32_BF16P])
 def test_pcg_mixed_reaches_fp64_tolerance(dev, prec):

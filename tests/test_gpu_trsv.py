@@ -141,6 +141,27 @@ def test_ic_apply(dev, mode, lvl):
 
 
 @pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("lvl", [0, 2])
+def test_ic_k_additive_apply(dev, mode, lvl):
+    """The incomplete_cholesky_k factor (additive shift, its own row order;
+    incomplete_cholesky.cpp:195-274) made by the product's host phase, then
+    applied on the device as U^-1 U^-T (:21-29)."""
+    import torch
+    A = _o(la.gen.laplace3d(12))
+    U = orc.split_triangle(A, True)
+    R, s = la.ick_shifted(as_lib_csr(U), lvl)
+    Ro, ok = orc.ick(U, lvl, shift=s)
+    assert ok and s == 0.0
+    np.testing.assert_array_equal(R.av, Ro.av)
+    b = la.gen.uniform(A.n, 8)
+    P = la.Preconditioner(R, "ic")
+    z = torch.zeros(A.n, dtype=torch.float64, device=dev)
+    P.apply(torch.tensor(b, device=dev), z, mode=mode)
+    ref = orc.prec_apply(A, "ic", Ro, b)
+    assert _close(z.cpu().numpy(), ref, 1e-11)
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_ilu_apply_transpose(dev, mode):
     import torch
     F = orc.iluk(_o(la.gen.powerlaw(2000, 8.0, 1.0, 300, seed=4)), 1)[0]
