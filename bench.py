@@ -550,7 +550,7 @@ def main():
     ap.add_argument("--c4-grid", type=int, default=160)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU sharding of the "
+        print("bench.py: runs on one GPU only; the multi-GPU sharding of the "
               "path is a design note (row-block halo exchange), not a measured mode.",
               file=sys.stderr)
         sys.exit(2)

@@ -456,6 +456,10 @@ class mkl_sparse_mat {
     detail::check(la_dtad(_nrow, _base, _ai.get(), _aj.get(), _av.get(), diag.data()));
     _dev.reset();
   }
+  // mkl_sparse_mat::to_one_based / to_zero_based (mkl_sparse_mat.cpp:235-267):
+  // shift the index arrays in place; the device copy is rebuilt on next use
+  void to_one_based() { rebase(1); }
+  void to_zero_based() { rebase(0); }
   // preconditioners override this (z = M^-1 r)
   virtual bool solve(double const *const, double *const) { return false; }
   virtual bool solve_device(double const *const, double *const) { return false; }
@@ -467,6 +471,14 @@ class mkl_sparse_mat {
   }
 
  protected:
+  void rebase(int b) {
+    if (_base == b || !_ai) return;
+    const int d = b - _base;
+    for (int i = 0; i < _nnz; ++i) _aj[i] += d;
+    for (int i = 0; i <= _nrow; ++i) _ai[i] += d;
+    _base = b;
+    _dev.reset();
+  }
   int _nrow = 0, _ncol = 0, _base = 0;
   std::shared_ptr<int[]> _ai, _aj;
   std::shared_ptr<double[]> _av;
@@ -518,7 +530,10 @@ class device_preconditioner : public mkl_sparse_mat {
   void set_trsv_mode(la_trsv_mode m) { trsv_mode_ = m; }
   la_trsv_mode trsv_mode() const { return trsv_mode_; }
   // sweeps per triangle of LA_TRSV_JACOBI applies (no counterpart in the reference)
-  void set_sweeps(int k) { detail::check(la_prec_set_sweeps(P_.get(), k)); }
+  void set_sweeps(int k) {
+    detail::check(la_prec_set_sweeps(P_.get(), k));
+    sweeps_ = k;
+  }
 
  protected:
   void upload_factor(bool ic) {
@@ -526,8 +541,10 @@ class device_preconditioner : public mkl_sparse_mat {
     if (ic) detail::check(la_prec_create_ic(_nrow, _base, _ai.get(), _aj.get(), _av.get(), &p));
     else detail::check(la_prec_create_ilu(_nrow, _base, _ai.get(), _aj.get(), _av.get(), &p));
     P_ = std::shared_ptr<la_prec_s>(p, [](la_prec_t q) { if (q) la_prec_destroy(q); });
+    if (sweeps_ > 0) detail::check(la_prec_set_sweeps(p, sweeps_));
   }
   std::shared_ptr<la_prec_s> P_;
+  int sweeps_ = 0;  // 0: the library's default
   la_trsv_mode trsv_mode_ = LA_TRSV_SYNCFREE;
 };
 
@@ -552,7 +569,10 @@ class incomplete_lu_k : public device_preconditioner {
     _nnz = static_cast<int>(nnz);
     return true;
   }
+  // The factor is left 1-based like the reference's (incomplete_lu.cpp:320-323);
+  // a later call first brings the pattern back to A's base.
   bool numeric_factorize(mkl_sparse_mat const *const A) {
+    rebase(A->mkl_base());
     const int st = la_iluk_numeric(_nrow, _base, A->get_ai().get(), A->get_aj().get(),
                                    A->get_av().get(), _ai.get(), _aj.get(), first_upper_.data(),
                                    _av.get());
@@ -562,6 +582,7 @@ class incomplete_lu_k : public device_preconditioner {
     }
     detail::check(st);
     upload_factor(false);
+    to_one_based();
     return true;
   }
 
@@ -570,7 +591,8 @@ class incomplete_lu_k : public device_preconditioner {
   std::vector<int> first_upper_;
 };
 
-// Symbolic phase shared by both IC(k) classes: the level-of-fill pattern of
+// Both IC(k) classes leave the factor 1-based like the reference's
+// (incomplete_cholesky.cpp:276-279, :337-340). Symbolic phase shared by both: the level-of-fill pattern of
 // the upper triangle of A (incomplete_cholesky.cpp:286-313 via
 // ICCLevelSymbolic; incomplete_cholesky_k's own symbolic phase at :74-193
 // yields the same pattern, tests/precond_test.cpp:63-116).
@@ -616,6 +638,7 @@ class incomplete_cholesky_common : public device_preconditioner {
 class incomplete_cholesky_k_2 : public incomplete_cholesky_common {
  public:
   bool numeric_factorize(mkl_sparse_mat const *const A) {
+    rebase(A->mkl_base());
     double shift = 0.0;
     for (int iter = 0;; ) {
       int ok = 0;
@@ -628,6 +651,7 @@ class incomplete_cholesky_k_2 : public incomplete_cholesky_common {
       std::printf("shift: %g\n", shift);
     }
     upload_factor(true);
+    to_one_based();
     return true;
   }
 };
@@ -639,6 +663,7 @@ class incomplete_cholesky_k_2 : public incomplete_cholesky_common {
 class incomplete_cholesky_k : public incomplete_cholesky_common {
  public:
   bool numeric_factorize(mkl_sparse_mat const *const A) {
+    rebase(A->mkl_base());
     const double *av = A->get_av().get();
     double shift = 0.0;
     if (_shift) {
@@ -658,6 +683,7 @@ class incomplete_cholesky_k : public incomplete_cholesky_common {
       std::printf("shift: %g\n", shift);
     }
     upload_factor(true);
+    to_one_based();
     return true;
   }
 };
@@ -678,7 +704,7 @@ class mkl_iterative_solver : public mkl_solver {
   void set_max_iters(int n) { _maxiter = n; }
   void set_rel_tol(double tol) { _rel_tol = tol; }
   void set_abs_tol(double tol) { _abs_tol = tol; }
-  // extensions of this build
+  // extensions with no counterpart in the reference
   void set_precision(la_precision p) { _precision = p; }
   void set_stop_rule(la_stop_rule r) { _stop = r; }
   void set_inner_tol(double t) { _inner_tol = t; }

@@ -3,9 +3,9 @@
  * reference algorithms (miaodi/linearAlgebra). TEST INFRASTRUCTURE ONLY.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
- * load this library, and only as the checker (or the timed CPU baseline),
- * never as the thing measured or shipped. The product path
- * (linearalgebra_amd) never links or calls it.
+ * load this library, and only as the result to compare against (or the
+ * timed CPU baseline), never as the thing measured or shipped. The product
+ * path (linearalgebra_amd) never links or calls it.
  *
  * Built with -O2 -ffp-contract=off: the reference's Release build has no
  * -march flag, so x86-64 baseline code carries no FMA and every product is

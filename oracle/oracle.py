@@ -1,7 +1,8 @@
 """ctypes front end of the CPU oracle (oracle/oracle.c).
 
 TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg use it as the checker or the timed CPU baseline. The
+cpu_baseline leg use it, as the result to compare against or as the
+timed CPU baseline. The
 product package (linearalgebra_amd) never imports it.
 
 Every wrapper takes/returns numpy arrays (int32 indices, float64 values) and

@@ -101,6 +101,8 @@ int main() {
   prec->shift(true);
   bool ok = prec->symbolic_factorize(&Asym) && prec->numeric_factorize(&Asym);
   expect(ok, "incomplete_cholesky_k factorize", 0);
+  expect(prec->mkl_base() == 1 && prec->get_ai()[0] == 1, "IC factor left 1-based",
+         prec->get_ai()[0]);
   mkl_wrapper::mkl_pcg_solver pcg(&Asym, prec);
   pcg.set_rel_tol(1e-10);
   pcg.set_max_iters(1000);
@@ -136,6 +138,18 @@ int main() {
       dmax = std::max(dmax, std::fabs(a - c) / std::fabs(c));
     }
     expect(dmax < 1e-13, "incomplete_cholesky_k_2 vs incomplete_cholesky_k", dmax);
+  }
+  // a second numeric phase on the 1-based factor (new values, same pattern)
+  // gives the same factor; to_zero_based shifts the indices back
+  {
+    std::vector<double> v0(prec->get_av().get(), prec->get_av().get() + prec->nnz());
+    ok = prec->numeric_factorize(&Asym);
+    bool same = ok && prec->get_ai()[0] == 1;
+    for (int q = 0; same && q < prec->nnz(); ++q) same = prec->get_av()[q] == v0[q];
+    expect(same, "numeric_factorize again on the 1-based factor", 0);
+    prec->to_zero_based();
+    expect(prec->mkl_base() == 0 && prec->get_ai()[0] == 0 && prec->get_aj()[0] == 0,
+           "to_zero_based", prec->get_ai()[0]);
   }
   // additive shift (incomplete_cholesky.cpp:207-266): [[-1, .5], [.5, 2]]
   // starts at s = 1e-3 + 1, fails on the second pivot, retries at 2.002;

@@ -9,7 +9,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from oracle import oracle as orc  # noqa: E402  (the checker)
+from oracle import oracle as orc  # noqa: E402  (CPU oracle)
 
 
 def read_mtx(name, base=0):

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/oracle.c) before tests compare against it.
 
 Each check is chosen so that a plausible mistake (a dropped term, a wrong
 sign or index, a transposed operand) fails: known-answer tests copied as
