@@ -66,7 +66,7 @@ struct Bfs {
     int h = 0;
     bool cut = false;
     const int nt = T.size();
-    Barrier bar(nt);
+    Barrier bar(T);
     std::vector<int64_t> sizes(nt);
     for (auto &row : bkt)
       for (auto &bk : row) bk.clear();
@@ -350,7 +350,7 @@ void rcm(int n, int base, const int *ai, const int *aj, int *iperm, int *perm, i
     int *out = iperm + off;  // the component's order, 0-based until the end
     out[0] = source - base;
     pos[source - base] = 0;
-    Barrier bar(T.size());
+    Barrier bar(T);
     T.run([&](int t, int k) {
       for (int64_t l = 0; l + 1 < L; ++l) {
         const int64_t pb = lp[l], pn = lp[l + 1] - lp[l];  // level l: out[pb .. pb+pn)

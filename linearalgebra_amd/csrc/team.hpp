@@ -6,15 +6,24 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
+#include <exception>
 #include <functional>
+#include <mutex>
 #include <thread>
 #include <vector>
 
 namespace laamd {
 namespace team {
 
+// Thrown out of Barrier::wait in the other members once one member's f has
+// thrown, so every member leaves the run instead of waiting for it.
+struct Aborted {};
+
 // A team of threads for the parallel phases; the caller is member 0.
 // run(f) calls f(t, nt) on every member and returns when all have finished.
+// If some member's f throws, the run is aborted (members waiting on a
+// Barrier of this team leave it) and run() rethrows the first exception
+// once every member is out; the team stays usable.
 class Team {
  public:
   explicit Team(int nt) : nt_(std::max(1, nt)) {
@@ -26,20 +35,36 @@ class Team {
     for (auto &t : th_) t.join();
   }
   int size() const { return nt_; }
+  const std::atomic<bool> *abort_flag() const { return &abort_; }
   void run(const std::function<void(int, int)> &f) {
     if (nt_ == 1) {
       f(0, 1);
       return;
     }
     job_ = &f;
+    err_ = nullptr;
+    abort_.store(false, std::memory_order_relaxed);
     done_.store(0, std::memory_order_relaxed);
     gen_.fetch_add(1, std::memory_order_release);
-    f(0, nt_);
+    call(0);
     for (int spins = 0; done_.load(std::memory_order_acquire) != nt_ - 1;)
       if (++spins > 256) std::this_thread::yield();
+    if (err_) std::rethrow_exception(err_);
   }
 
  private:
+  void call(int t) {
+    try {
+      (*job_)(t, nt_);
+    } catch (const Aborted &) {
+    } catch (...) {
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        if (!err_) err_ = std::current_exception();
+      }
+      abort_.store(true, std::memory_order_release);
+    }
+  }
   void worker(int t) {
     uint64_t seen = 0;
     for (;;) {
@@ -48,7 +73,7 @@ class Team {
         if (++spins > 256) std::this_thread::yield();
       seen = g;
       if (quit_.load(std::memory_order_relaxed)) return;
-      (*job_)(t, nt_);
+      call(t);
       done_.fetch_add(1, std::memory_order_release);
     }
   }
@@ -57,14 +82,17 @@ class Team {
   const std::function<void(int, int)> *job_ = nullptr;
   std::atomic<uint64_t> gen_{0};
   std::atomic<int> done_{0};
-  std::atomic<bool> quit_{false};
+  std::atomic<bool> quit_{false}, abort_{false};
+  std::mutex mu_;
+  std::exception_ptr err_;
 };
 
 // Sense-reversing barrier for the members of one Team::run (spins, then
-// yields, so an oversubscribed host still makes progress).
+// yields, so an oversubscribed host still makes progress). Throws Aborted
+// when the team's run has been aborted.
 class Barrier {
  public:
-  explicit Barrier(int nt) : nt_(nt) {}
+  explicit Barrier(const Team &T) : nt_(T.size()), abort_(T.abort_flag()) {}
   void wait() {
     if (nt_ == 1) return;
     const int gen = gen_.load(std::memory_order_acquire);
@@ -73,12 +101,15 @@ class Barrier {
       gen_.fetch_add(1, std::memory_order_release);
       return;
     }
-    for (int spins = 0; gen_.load(std::memory_order_acquire) == gen;)
+    for (int spins = 0; gen_.load(std::memory_order_acquire) == gen;) {
+      if (abort_->load(std::memory_order_acquire)) throw Aborted{};
       if (++spins > (1 << 16)) std::this_thread::yield();
+    }
   }
 
  private:
   int nt_;
+  const std::atomic<bool> *abort_;
   std::atomic<int> count_{0}, gen_{0};
 };
 
