@@ -277,6 +277,13 @@ int la_prec_create_ilu_gpu(int n, int base, const int *ai, const int *aj, int le
                            la_prec_t *out);
 int la_prec_create_ic_gpu(int n, int base, const int *ai, const int *aj, int level, double omega,
                           la_prec_t *out);
+/* The same with incomplete_cholesky_k's numeric arithmetic
+ * (mkl_wrapper/incomplete_cholesky.cpp:195-274, one pass of its retry
+ * loop): `shift` is ADDED to the diagonal, rows are eliminated in that
+ * class's first-in first-out column order, and a pivot <= 0 gives
+ * LA_ERR_NUMERIC; the factor is bitwise la_ick_numeric's. */
+int la_prec_create_ick_gpu(int n, int base, const int *ai, const int *aj, int level,
+                           double shift, la_prec_t *out);
 int la_prec_refactor(la_prec_t P, const double *a_vals);
 int la_prec_factor(la_prec_t P, int *fai, int *faj, double *fav, int64_t *nnz);
 int la_prec_destroy(la_prec_t P);

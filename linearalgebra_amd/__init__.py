@@ -249,6 +249,17 @@ class Preconditioner:
                                           C.byref(h)))
         return cls(A.n, "ic", _handle=h)
 
+    @classmethod
+    def ick_gpu(cls, A, level, shift=0.0):
+        """IC(level) of the symmetric A with incomplete_cholesky_k's numeric
+        arithmetic (additive shift, first-in first-out rows, a pivot <= 0
+        fails; incomplete_cholesky.cpp:195-274), on the device
+        (la_prec_create_ick_gpu)."""
+        h = C.c_void_p()
+        check(lib().la_prec_create_ick_gpu(A.n, A.base, _p(A.ai), _p(A.aj), level, float(shift),
+                                           C.byref(h)))
+        return cls(A.n, "ic", _handle=h)
+
     def refactor(self, a_vals):
         """Numeric factorisation on the device from A's FP64 values (a
         device tensor in the CSR order given at creation)."""

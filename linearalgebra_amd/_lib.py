@@ -99,6 +99,7 @@ _SIGS = {
     "la_prec_create_ic": [_i, _i, _vp, _vp, _vp, C.POINTER(_vp)],
     "la_prec_create_ilu_gpu": [_i, _i, _vp, _vp, _i, C.POINTER(_vp)],
     "la_prec_create_ic_gpu": [_i, _i, _vp, _vp, _i, _d, C.POINTER(_vp)],
+    "la_prec_create_ick_gpu": [_i, _i, _vp, _vp, _i, _d, C.POINTER(_vp)],
     "la_prec_refactor": [_vp, _vp],
     "la_prec_factor": [_vp, _vp, _vp, _vp, C.POINTER(_i64)],
     "la_prec_destroy": [_vp],

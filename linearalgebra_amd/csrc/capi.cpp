@@ -491,6 +491,16 @@ int la_prec_create_ic_gpu(int n, int base, const int *ai, const int *aj, int lev
   *out = h.release();
   LA_CATCH
 }
+int la_prec_create_ick_gpu(int n, int base, const int *ai, const int *aj, int level, double shift,
+                           la_prec_t *out) {
+  LA_TRY
+  LA_CHECK_ARG(out && ai && (base == 0 || base == 1) && n >= 0 && level >= 0, "bad argument");
+  LA_CHECK_ARG(ai[0] == base && (ai[n] == base || aj), "bad CSR");
+  auto h = std::make_unique<la_prec_s>();
+  h->p = make_prec_ic_gpu(n, base, ai, aj, level, shift, nullptr, true);
+  *out = h.release();
+  LA_CATCH
+}
 int la_prec_refactor(la_prec_t P, const double *a_vals) {
   LA_TRY
   LA_CHECK_ARG(P, "null handle");

@@ -146,8 +146,10 @@ std::unique_ptr<Prec> make_prec_ic(int n, int base, const int *rai, const int *r
 // factor) on the device at each prec_refactor
 std::unique_ptr<Prec> make_prec_ilu_gpu(int n, int base, const int *ai, const int *aj, int level,
                                         hipStream_t s);
+// additive: incomplete_cholesky_k's arithmetic (omega is then the added
+// shift, rows first-in first-out, a pivot <= 0 fails)
 std::unique_ptr<Prec> make_prec_ic_gpu(int n, int base, const int *ai, const int *aj, int level,
-                                       double omega, hipStream_t s);
+                                       double omega, hipStream_t s, bool additive = false);
 // a_vals: device FP64 values of A in the CSR layout given at creation
 void prec_refactor(Prec &P, const double *a_vals, hipStream_t s);
 // factor values (ILU: combined LU; IC: upper R) and pattern, host copies

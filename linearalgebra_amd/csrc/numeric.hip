@@ -12,7 +12,12 @@
 //    pattern): the diagonal is scaled by (1 + omega); for each row k with an
 //    entry in column j, in the reference's linked-list order (recorded on
 //    the host once per pattern), u_j* -= u_kj * u_k* over the merged
-//    columns; then u_jj = sqrt(pivot) and the row is divided by it.
+//    columns; then u_jj = sqrt(pivot) and the row is divided by it; a
+//    pivot < 0 fails.
+//  * IC with the additive shift (mkl_wrapper/incomplete_cholesky.cpp:
+//    195-274, incomplete_cholesky_k): the same updates with the shift added
+//    to the diagonal, the rows k of each column taken first-in first-out
+//    (that class's column vectors), and a pivot <= 0 failing.
 // Products and differences are rounded separately (no fused multiply-add),
 // as in the reference's x86-64 build.
 //
@@ -42,6 +47,7 @@ struct NumericPlan {
   int n = 0;
   int64_t nnzA = 0, nnzF = 0;
   double omega = 0.0;
+  bool additive = false;   // IC: incomplete_cholesky_k's arithmetic, omega = the shift
   DevBuf fai, faj, fu;     // pattern (0-based); ILU: first upper position per row
   DevBuf a2f;              // per entry of A: its position in the factor, or -1
   DevBuf fav, diag;        // factor values; ILU: u_ii per row
@@ -68,6 +74,10 @@ __global__ void k_scatter(int64_t m, const int64_t *__restrict__ a2f,
 __global__ void k_scale_first(int n, const int *__restrict__ fai, double *fav, double f) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n && fai[j + 1] > fai[j]) fav[fai[j]] = fav[fai[j]] * f;
+}
+__global__ void k_shift_first(int n, const int *__restrict__ fai, double *fav, double a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n && fai[j + 1] > fai[j]) fav[fai[j]] = fav[fai[j]] + a;
 }
 
 __global__ void k_fill_zero(int64_t m, double *v) {
@@ -157,8 +167,9 @@ __global__ __launch_bounds__(64) void k_ilu_rows(int k0, const int *__restrict__
 }
 
 // one row of IC by one lane, in global memory
+// (strict: a zero pivot fails too)
 __device__ void ic_row_serial(int j, const int *fai, const int *faj, const int *pptr,
-                              const int *pk, const int *pq, double *fav, int *err) {
+                              const int *pk, const int *pq, double *fav, int *err, int strict) {
   const int ps = fai[j], pe = fai[j + 1];
   for (int e = pptr[j]; e < pptr[j + 1]; ++e) {
     const int k = pk[e];
@@ -178,7 +189,7 @@ __device__ void ic_row_serial(int j, const int *fai, const int *faj, const int *
       }
     }
   }
-  if (fav[ps] < 0) {
+  if (strict ? fav[ps] <= 0 : fav[ps] < 0) {
     flag_row(err, j);
     return;
   }
@@ -193,14 +204,14 @@ __global__ __launch_bounds__(64) void k_ic_rows(int k0, const int *__restrict__ 
                                                 const int *__restrict__ pptr,
                                                 const int *__restrict__ pk,
                                                 const int *__restrict__ pq, double *fav,
-                                                int *err) {
+                                                int *err, int strict) {
   __shared__ int cols[kRowCap];
   __shared__ double vals[kRowCap];
   const int j = rows[k0 + blockIdx.x];
   const int lane = threadIdx.x;
   const int ps = fai[j], len = fai[j + 1] - ps;
   if (len > kRowCap) {
-    if (lane == 0) ic_row_serial(j, fai, faj, pptr, pk, pq, fav, err);
+    if (lane == 0) ic_row_serial(j, fai, faj, pptr, pk, pq, fav, err, strict);
     return;
   }
   for (int t = lane; t < len; t += 64) {
@@ -219,7 +230,7 @@ __global__ __launch_bounds__(64) void k_ic_rows(int k0, const int *__restrict__ 
     __syncthreads();
   }
   const double piv = vals[0];
-  if (piv < 0) {
+  if (strict ? piv <= 0 : piv < 0) {
     if (lane == 0) flag_row(err, j);
     return;
   }
@@ -334,7 +345,7 @@ std::unique_ptr<Prec> make_prec_ilu_gpu(int n, int base, const int *ai, const in
 }
 
 std::unique_ptr<Prec> make_prec_ic_gpu(int n, int base, const int *ai, const int *aj, int level,
-                                       double omega, hipStream_t s) {
+                                       double omega, hipStream_t s, bool additive) {
   LA_CHECK_ARG(n >= 0 && level >= 0, "bad size or level");
   auto P = std::make_unique<Prec>();
   P->ic = true;
@@ -345,6 +356,7 @@ std::unique_ptr<Prec> make_prec_ic_gpu(int n, int base, const int *ai, const int
   N.ic = true;
   N.n = n;
   N.omega = omega;
+  N.additive = additive;
   N.nnzA = static_cast<int64_t>(ai[n]) - base;
   // upper part of A with the diagonal first in each row
   std::vector<int> ui(static_cast<size_t>(n) + 1, 0), uj, dpos(static_cast<size_t>(n));
@@ -367,9 +379,10 @@ std::unique_ptr<Prec> make_prec_ic_gpu(int n, int base, const int *ai, const int
   icc_symbolic(n, 0, ui.data(), uj.data(), dpos.data(), level, fai, faj);
   N.nnzF = fai[n];
   std::vector<int64_t> a2f = map_a_to_f(n, ai, aj, base, fai, faj, true);
-  // the reference's elimination order per row (icc_numeric's list walk)
+  // the reference's elimination order per row: icc_numeric's list walk
+  // (last-in first-out) or ick_numeric's column queues (first-in first-out)
   std::vector<int> pptr(static_cast<size_t>(n) + 1, 0), pk, pq;
-  {
+  if (!additive) {
     const int NONE = INT_MAX;
     std::vector<int> llist(static_cast<size_t>(n), NONE);
     std::vector<int64_t> jk(static_cast<size_t>(n), 0);
@@ -393,6 +406,37 @@ std::unique_ptr<Prec> make_prec_ic_gpu(int n, int base, const int *ai, const int
         llist[j] = llist[c];
         llist[c] = j;
         jk[j] = ps + 1;
+      }
+      pptr[j + 1] = static_cast<int>(pk.size());
+    }
+  } else {
+    const int NONE = -1;
+    std::vector<int> head(static_cast<size_t>(n), NONE), tail(static_cast<size_t>(n), NONE),
+        next(static_cast<size_t>(n), NONE);
+    std::vector<int64_t> pos(static_cast<size_t>(n), 0);
+    auto enqueue = [&](int k, int c) {
+      next[k] = NONE;
+      if (head[c] == NONE) head[c] = k;
+      else next[tail[c]] = k;
+      tail[c] = k;
+    };
+    for (int j = 0; j < n; ++j) {
+      const int64_t ps = fai[j], pe = fai[j + 1];
+      for (int k = head[j]; k != NONE;) {
+        const int knext = next[k];
+        const int64_t q = pos[k];
+        if (q + 1 < fai[k + 1]) {
+          pos[k] = q + 1;
+          enqueue(k, faj[q + 1]);
+        }
+        pk.push_back(k);
+        pq.push_back(static_cast<int>(q));
+        k = knext;
+      }
+      head[j] = NONE;
+      if (pe - ps > 1) {
+        pos[j] = ps + 1;
+        enqueue(j, faj[ps + 1]);
       }
       pptr[j + 1] = static_cast<int>(pk.size());
     }
@@ -429,7 +473,10 @@ void prec_refactor(Prec &P, const double *a_vals, hipStream_t s) {
   if (N.nnzA)
     hipLaunchKernelGGL(k_scatter, dim3(static_cast<unsigned>((N.nnzA + 255) / 256)), dim3(256),
                        0, s, N.nnzA, N.a2f.as<int64_t>(), a_vals, fav);
-  if (N.ic)
+  if (N.ic && N.additive)
+    hipLaunchKernelGGL(k_shift_first, dim3((N.n + 255) / 256), dim3(256), 0, s, N.n,
+                       N.fai.as<int>(), fav, N.omega);
+  else if (N.ic)
     hipLaunchKernelGGL(k_scale_first, dim3((N.n + 255) / 256), dim3(256), 0, s, N.n,
                        N.fai.as<int>(), fav, 1.0 + N.omega);
   LA_HIP(hipGetLastError());
@@ -440,7 +487,7 @@ void prec_refactor(Prec &P, const double *a_vals, hipStream_t s) {
     if (N.ic)
       hipLaunchKernelGGL(k_ic_rows, dim3(cnt), dim3(64), 0, s, k0, N.rows.as<int>(),
                          N.fai.as<int>(), N.faj.as<int>(), N.pptr.as<int>(), N.pk.as<int>(),
-                         N.pq.as<int>(), fav, N.err.as<int>());
+                         N.pq.as<int>(), fav, N.err.as<int>(), N.additive ? 1 : 0);
     else
       hipLaunchKernelGGL(k_ilu_rows, dim3(cnt), dim3(64), 0, s, k0, N.rows.as<int>(),
                          N.fai.as<int>(), N.faj.as<int>(), N.fu.as<int>(), fav,
@@ -451,8 +498,9 @@ void prec_refactor(Prec &P, const double *a_vals, hipStream_t s) {
   LA_HIP(hipMemcpyAsync(&bad, N.err.p, sizeof(int), hipMemcpyDeviceToHost, s));
   LA_HIP(hipStreamSynchronize(s));
   if (bad)
-    throw la_error(LA_ERR_NUMERIC, N.ic ? "IC factorisation hit a negative pivot"
-                                        : "ILU factorisation met a row without a diagonal");
+    throw la_error(LA_ERR_NUMERIC, !N.ic        ? "ILU factorisation met a row without a diagonal"
+                                   : N.additive ? "IC factorisation hit a non-positive pivot"
+                                                : "IC factorisation hit a negative pivot");
   trsv_refresh(*P.lower, fav, s);
   trsv_refresh(*P.upper, fav, s);
   LA_HIP(hipStreamSynchronize(s));

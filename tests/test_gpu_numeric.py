@@ -86,6 +86,53 @@ def test_ic_gpu_bitwise_host(dev, name, make, level):
         assert rel_err(z.cpu().numpy(), zr) < 1e-12
 
 
+@pytest.mark.parametrize("name,make", _ic_cases(), ids=[c[0] for c in _ic_cases()])
+@pytest.mark.parametrize("level", [0, 1, 2])
+@pytest.mark.parametrize("shift", [0.0, 0.125])
+def test_ick_gpu_bitwise_host(dev, name, make, level, shift):
+    """incomplete_cholesky_k's numeric phase on the device (additive shift,
+    first-in first-out rows; incomplete_cholesky.cpp:195-274) is bitwise the
+    host's la_ick_numeric, which is bitwise the oracle's (tests/test_ick.py);
+    a pivot <= 0 is reported by both."""
+    import torch
+    A = make()
+    U = la.split_triangle(A, upper=True)
+    dp = U.ai[:-1]
+    ia, ja = la.icc_symbolic(U, dp, level)
+    R, ok = la.ick_numeric(U, dp, ia, ja, shift)
+    P = la.Preconditioner.ick_gpu(A, level, shift)
+    if not ok:
+        with pytest.raises(la.LaError) as e:
+            P.refactor(torch.tensor(A.av, device=dev))
+        assert e.value.code == 5
+        return
+    P.refactor(torch.tensor(A.av, device=dev))
+    G = P.factor()
+    assert np.array_equal(G.ai, np.asarray(R.ai) - R.base)
+    assert np.array_equal(G.aj, np.asarray(R.aj) - R.base)
+    assert np.array_equal(G.av, R.av), "max diff %.3e" % np.max(np.abs(G.av - R.av))
+    r = la.gen.uniform(A.n, 5)
+    zr = orc.prec_apply(_o(A), "ic", _o(R), r)
+    rt = torch.tensor(r, device=dev)
+    z = torch.zeros_like(rt)
+    P.apply(rt, z, mode=la.TRSV_SYNCFREE)
+    assert rel_err(z.cpu().numpy(), zr) < 1e-12
+
+
+def test_ick_gpu_zero_pivot(dev):
+    """[[1,1],[1,1]]: the second pivot is exactly 0, which
+    incomplete_cholesky_k rejects (:259) and ICCLevelNumeric accepts
+    (precond.hpp:683)."""
+    import torch
+    from helpers import from_dense
+    A = as_lib_csr(from_dense(np.array([[1.0, 1.0], [1.0, 1.0]])))
+    vals = torch.tensor(A.av, device=dev)
+    with pytest.raises(la.LaError) as e:
+        la.Preconditioner.ick_gpu(A, 0).refactor(vals)
+    assert e.value.code == 5
+    la.Preconditioner.ic_gpu(A, 0).refactor(vals)
+
+
 def test_ic_gpu_shift_and_refactor(dev):
     """Two refactors in a row with different values; the multiplicative
     shift (1 + omega) of ICCLevelNumeric (precond.hpp:636-638)."""
