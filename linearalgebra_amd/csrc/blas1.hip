@@ -150,6 +150,64 @@ __device__ __forceinline__ V &vel(T &a, int e) {
 constexpr int kUn = 4;  // vectors in flight per operand and thread
 inline bool al16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
+// <V_i, w> for i < k with 16-byte loads: the basis vectors are taken kMdG
+// at a time, each thread reading w once per group and the group's vectors
+// beside it (kMdG + 1 loads in flight), one FP64 accumulator per vector.
+// Block b writes part[i * gridDim.x + b] (the layout k_reduce_partials
+// sums); the order of every sum is fixed by n and the grid. Needs V, w and
+// the stride ld 16-byte aligned.
+constexpr int kMdG = 8;
+template <typename V>
+__global__ __launch_bounds__(kB) void k_multidot_partial_v(int64_t n, int k,
+                                                           const V *__restrict__ Vm, int64_t ld,
+                                                           const V *__restrict__ w,
+                                                           double *part) {
+  using T = typename V16<V>::T;
+  constexpr int K = V16<V>::k;
+  __shared__ double sh[kMdG][kB / 64];
+  const T *wv = reinterpret_cast<const T *>(w);
+  const int64_t nv = n / K, stride = static_cast<int64_t>(gridDim.x) * kB;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i0 = 0; i0 < k; i0 += kMdG) {
+    const int g = k - i0 < kMdG ? k - i0 : kMdG;
+    double acc[kMdG];
+#pragma unroll
+    for (int e = 0; e < kMdG; ++e) acc[e] = 0;
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; v < nv; v += stride) {
+      T b = wv[v];
+      T a[kMdG];
+#pragma unroll
+      for (int e = 0; e < kMdG; ++e)
+        if (e < g) a[e] = reinterpret_cast<const T *>(Vm + static_cast<int64_t>(i0 + e) * ld)[v];
+#pragma unroll
+      for (int e = 0; e < kMdG; ++e)
+        if (e < g)
+#pragma unroll
+          for (int c = 0; c < K; ++c)
+            acc[e] += static_cast<double>(vel<T, V>(a[e], c)) *
+                      static_cast<double>(vel<T, V>(b, c));
+    }
+    if (blockIdx.x == 0 && threadIdx.x < n - nv * K) {
+      const int64_t t = nv * K + threadIdx.x;
+      for (int e = 0; e < g; ++e)
+        acc[e] += static_cast<double>(Vm[static_cast<int64_t>(i0 + e) * ld + t]) *
+                  static_cast<double>(w[t]);
+    }
+#pragma unroll
+    for (int e = 0; e < kMdG; ++e) {
+      const double r = wave_sum(acc[e]);
+      if (lane == 0) sh[e][wave] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < g) {
+      double t = 0;
+      for (int q = 0; q < kB / 64; ++q) t += sh[threadIdx.x][q];
+      part[static_cast<int64_t>(i0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
 template <typename V>
 __global__ __launch_bounds__(kB) void k_dot_partial_v(int64_t n, const V *__restrict__ x,
                                                       const V *__restrict__ y, double *part) {
@@ -540,7 +598,15 @@ void dot_dev(int64_t n, la_dtype vec, const void *x, const void *y, double *part
 void multidot_dev(int64_t n, int k, la_dtype vec, const void *V, int64_t ld, const void *w,
                   double *partials, double *out, hipStream_t s) {
   LA_CHECK_ARG(k <= kMaxDots, "too many dot products");
-  if (vec == LA_F64)
+  const size_t es = vec == LA_F64 ? sizeof(double) : sizeof(float);
+  const bool v16 = al16(V) && al16(w) && (static_cast<size_t>(ld) * es) % 16 == 0;
+  if (vec == LA_F64 && v16)
+    hipLaunchKernelGGL(k_multidot_partial_v<double>, dim3(kRedBlocks), dim3(kB), 0, s, n, k,
+                       static_cast<const double *>(V), ld, static_cast<const double *>(w), partials);
+  else if (v16)
+    hipLaunchKernelGGL(k_multidot_partial_v<float>, dim3(kRedBlocks), dim3(kB), 0, s, n, k,
+                       static_cast<const float *>(V), ld, static_cast<const float *>(w), partials);
+  else if (vec == LA_F64)
     hipLaunchKernelGGL(k_multidot_partial<double>, dim3(kRedBlocks), dim3(kB), 0, s, n, k,
                        static_cast<const double *>(V), ld, static_cast<const double *>(w), partials);
   else
