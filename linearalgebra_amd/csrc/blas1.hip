@@ -584,13 +584,7 @@ double wait_report(Scratch &S, uint64_t seq, hipStream_t s) {
 
 void dot_dev(int64_t n, la_dtype vec, const void *x, const void *y, double *partials, double *out,
              hipStream_t s) {
-  if (vec == LA_F64)
-    hipLaunchKernelGGL(k_dot_partial<double>, dim3(kRedBlocks), dim3(kB), 0, s, n,
-                       static_cast<const double *>(x), static_cast<const double *>(y), partials);
-  else
-    hipLaunchKernelGGL(k_dot_partial<float>, dim3(kRedBlocks), dim3(kB), 0, s, n,
-                       static_cast<const float *>(x), static_cast<const float *>(y), partials);
-  LA_HIP(hipGetLastError());
+  dot_partials(n, vec, x, y, partials, s);  // 16-byte loads when x and y allow them
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kB), 0, s, 1, kRedBlocks, partials, out);
   LA_HIP(hipGetLastError());
 }

@@ -584,6 +584,7 @@ int la_dot(int64_t n, la_dtype vec, const void *x, const void *y, double *result
            void *stream) {
   LA_TRY
   LA_CHECK_ARG(n >= 0 && (vec == LA_F64 || vec == LA_F32) && result, "bad argument");
+  LA_CHECK_ARG(n == 0 || (x && y), "null vector");
   hipStream_t s = static_cast<hipStream_t>(stream);
   Scratch &S = blas_scratch();
   dot_dev(n, vec, x, y, S.partials.as<double>(), S.scalars.as<double>(), s);
@@ -597,6 +598,7 @@ int la_axpy_dot(int64_t n, la_dtype vec, double a, const void *x, void *y, const
                 double *result, void *stream) {
   LA_TRY
   LA_CHECK_ARG(n >= 0 && (vec == LA_F64 || vec == LA_F32) && result, "bad argument");
+  LA_CHECK_ARG(n == 0 || (x && y), "null vector");
   hipStream_t s = static_cast<hipStream_t>(stream);
   axpby(n, vec, a, x, 1.0, y, s);
   Scratch &S = blas_scratch();

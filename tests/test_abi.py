@@ -37,3 +37,11 @@ def test_error_reporting_without_gpu():
 def test_oracle_loads():
     from oracle import oracle as orc
     assert orc.max_threads() >= 1
+
+
+def test_blas1_null_vectors_rejected():
+    """la_dot / la_axpy_dot check their vectors before touching the device."""
+    r = C.c_double()
+    assert la.lib().la_dot(10, _lib.F64, None, None, C.byref(r), None) == _lib.LA_ERR_ARG
+    assert la.lib().la_axpy_dot(10, _lib.F64, 1.0, None, None, None, C.byref(r),
+                                None) == _lib.LA_ERR_ARG
