@@ -220,6 +220,7 @@ int la_icc_numeric(int n, int base, const int *ai, const int *aj, const double *
                    double *icc_av, int *ok) {
   LA_TRY
   LA_CHECK_ARG(n >= 0 && ai && diag_pos && icc_ai && icc_av, "null argument");
+  LA_CHECK_ARG(icc_ai[n] == icc_ai[0] || (aj && av && icc_aj), "null argument");
   const bool good = icc_numeric(n, base, ai, aj, av, diag_pos, omega, icc_ai, icc_aj, icc_av);
   if (ok) *ok = good ? 1 : 0;
   LA_CATCH
@@ -230,6 +231,7 @@ int la_ick_numeric(int n, int base, const int *ai, const int *aj, const double *
                    double *icc_av, int *ok) {
   LA_TRY
   LA_CHECK_ARG(n >= 0 && ai && diag_pos && icc_ai && icc_av, "null argument");
+  LA_CHECK_ARG(icc_ai[n] == icc_ai[0] || (aj && av && icc_aj), "null argument");
   const bool good = ick_numeric(n, base, ai, aj, av, diag_pos, shift, icc_ai, icc_aj, icc_av);
   if (ok) *ok = good ? 1 : 0;
   LA_CATCH
