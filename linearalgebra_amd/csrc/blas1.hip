@@ -208,6 +208,40 @@ __global__ __launch_bounds__(kB) void k_multidot_partial_v(int64_t n, int k,
   }
 }
 
+// w -= sum_i h_i V_i with 16-byte loads (same arithmetic per element as
+// k_multiaxpy: FP64 accumulation in index order, one rounding to V). Needs
+// V, w and the stride ld 16-byte aligned.
+template <typename V>
+__global__ __launch_bounds__(kB) void k_multiaxpy_v(int64_t n, int k, const V *__restrict__ Vm,
+                                                    int64_t ld, const double *__restrict__ h,
+                                                    V *w) {
+  using T = typename V16<V>::T;
+  constexpr int K = V16<V>::k;
+  T *wv = reinterpret_cast<T *>(w);
+  const int64_t nv = n / K, stride = static_cast<int64_t>(gridDim.x) * kB;
+  for (int64_t v = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; v < nv; v += stride) {
+    T b = wv[v];
+    double acc[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] = static_cast<double>(vel<T, V>(b, c));
+    for (int i = 0; i < k; ++i) {
+      T a = reinterpret_cast<const T *>(Vm + static_cast<int64_t>(i) * ld)[v];
+      const double hi = h[i];
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] -= hi * static_cast<double>(vel<T, V>(a, c));
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) vel<T, V>(b, c) = static_cast<V>(acc[c]);
+    wv[v] = b;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * K) {
+    const int64_t t = nv * K + threadIdx.x;
+    double acc = static_cast<double>(w[t]);
+    for (int i = 0; i < k; ++i) acc -= h[i] * static_cast<double>(Vm[static_cast<int64_t>(i) * ld + t]);
+    w[t] = static_cast<V>(acc);
+  }
+}
+
 template <typename V>
 __global__ __launch_bounds__(kB) void k_dot_partial_v(int64_t n, const V *__restrict__ x,
                                                       const V *__restrict__ y, double *part) {
@@ -613,6 +647,17 @@ void multidot_dev(int64_t n, int k, la_dtype vec, const void *V, int64_t ld, con
 
 void multiaxpy_dev(int64_t n, int k, la_dtype vec, const void *V, int64_t ld, const double *h,
                    void *w, hipStream_t s) {
+  const size_t es = vec == LA_F64 ? sizeof(double) : sizeof(float);
+  if (al16(V) && al16(w) && (static_cast<size_t>(ld) * es) % 16 == 0) {
+    if (vec == LA_F64)
+      hipLaunchKernelGGL(k_multiaxpy_v<double>, dim3(grid_for(n / 2)), dim3(kB), 0, s, n, k,
+                         static_cast<const double *>(V), ld, h, static_cast<double *>(w));
+    else
+      hipLaunchKernelGGL(k_multiaxpy_v<float>, dim3(grid_for(n / 4)), dim3(kB), 0, s, n, k,
+                         static_cast<const float *>(V), ld, h, static_cast<float *>(w));
+    LA_HIP(hipGetLastError());
+    return;
+  }
   if (vec == LA_F64)
     hipLaunchKernelGGL(k_multiaxpy<double>, dim3(grid_for(n)), dim3(kB), 0, s, n, k,
                        static_cast<const double *>(V), ld, h, static_cast<double *>(w));

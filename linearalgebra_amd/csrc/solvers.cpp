@@ -188,7 +188,10 @@ CgResult fgmres_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_
   double *h1 = sc + 8, *h2 = sc + 8 + kMaxDots, *hn = sc + 8 + 2 * kMaxDots;
   const la_spmv_alg alg = static_cast<la_spmv_alg>(o.spmv_alg);
   std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
-  std::vector<double> hbuf(2 * kMaxDots + 1);
+  // pinned staging for the Hessenberg column and the update coefficients
+  // (a pageable buffer would add a staged copy per iteration)
+  double *hbuf = W.sc.host + 16;
+  static_assert(16 + 2 * kMaxDots + 1 <= 2 * kRedBlocks, "pinned scratch too small");
   CgResult R;
   int total = 0;
   bool done = false;
@@ -213,7 +216,7 @@ CgResult fgmres_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_
       multidot_dev(n, j + 1, vec, V, ld, w, part, h2, s);
       multiaxpy_dev(n, j + 1, vec, V, ld, h2, w, s);
       dot_dev(n, vec, w, w, part, hn, s);
-      LA_HIP(hipMemcpyAsync(hbuf.data(), h1, sizeof(double) * (2 * kMaxDots + 1),
+      LA_HIP(hipMemcpyAsync(hbuf, h1, sizeof(double) * (2 * kMaxDots + 1),
                             hipMemcpyDeviceToHost, s));
       LA_HIP(hipStreamSynchronize(s));
       for (int i = 0; i <= j; ++i) H[static_cast<size_t>(i) * m + j] = hbuf[i] + hbuf[kMaxDots + i];
@@ -249,7 +252,7 @@ CgResult fgmres_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_
     }
     // x += Z y  (multiaxpy subtracts, so pass -y)
     for (int l = 0; l < k; ++l) hbuf[l] = -y[l];
-    LA_HIP(hipMemcpyAsync(h1, hbuf.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    LA_HIP(hipMemcpyAsync(h1, hbuf, sizeof(double) * k, hipMemcpyHostToDevice, s));
     multiaxpy_dev(n, k, vec, Z, n, h1, x, s);
     LA_HIP(hipStreamSynchronize(s));
   }
