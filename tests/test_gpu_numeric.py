@@ -30,6 +30,7 @@ def _ilu_cases():
 
 def _ic_cases():
     return [("ex5", lambda: as_lib_csr(read_mtx("ex5.mtx"))),
+            ("ex5_base1", lambda: as_lib_csr(read_mtx("ex5.mtx", base=1))),
             ("poisson2d_40", lambda: la.gen.poisson2d(40)),
             ("laplace3d_12", lambda: la.gen.laplace3d(12)),
             ("aniso27_10", lambda: la.gen.aniso27(10, 9, 8))]
