@@ -38,10 +38,10 @@ __global__ __launch_bounds__(kB) void k_dot_partial(int64_t n, const V *__restri
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// out[i] = sum_b part[i*nb + b], fixed order
+// out[i] = sum_b part[i*nb + b], fixed order; one workgroup per i
 __global__ void k_reduce_partials(int k, int nb, const double *__restrict__ part, double *out) {
   __shared__ double sh[kB / 64];
-  for (int i = 0; i < k; ++i) {
+  for (int i = blockIdx.x; i < k; i += gridDim.x) {
     double acc = 0;
     for (int b = threadIdx.x; b < nb; b += kB) acc += part[static_cast<int64_t>(i) * nb + b];
     acc = wave_sum(acc);
@@ -641,7 +641,8 @@ void multidot_dev(int64_t n, int k, la_dtype vec, const void *V, int64_t ld, con
     hipLaunchKernelGGL(k_multidot_partial<float>, dim3(kRedBlocks), dim3(kB), 0, s, n, k,
                        static_cast<const float *>(V), ld, static_cast<const float *>(w), partials);
   LA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kB), 0, s, k, kRedBlocks, partials, out);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(std::max(k, 1)), dim3(kB), 0, s, k, kRedBlocks,
+                     partials, out);
   LA_HIP(hipGetLastError());
 }
 
