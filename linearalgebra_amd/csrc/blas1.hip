@@ -338,6 +338,7 @@ __global__ __launch_bounds__(kB) void k_cg_update_xr_v(int64_t n, V *x, V *r,
     r[i] = rn;
     acc += static_cast<double>(rn) * static_cast<double>(rn);
   }
+  __syncthreads();  // every wave is past its reads of sh in block_sum_partials
   const double s = block_reduce<V>(acc, sh);
   if (threadIdx.x == 0) part_rr[blockIdx.x] = s;
 }
@@ -402,6 +403,7 @@ __global__ __launch_bounds__(kB) void k_cg_update_xr(int64_t n, V *x, V *r, cons
     r[t] = rn;
     acc += static_cast<double>(rn) * static_cast<double>(rn);
   }
+  __syncthreads();  // every wave is past its reads of sh in block_sum_partials
   const double s = block_reduce<V>(acc, sh);
   if (threadIdx.x == 0) part_rr[blockIdx.x] = s;
 }
