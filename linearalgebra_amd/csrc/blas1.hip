@@ -577,24 +577,30 @@ Scratch::~Scratch() {
 }
 
 namespace {
+// the slot's "no report yet" value: a signalling-NaN pattern that no sum
+// produces (arithmetic yields the canonical quiet NaN)
+constexpr uint64_t kRepEmpty = 0x7FF4DEADBEEF0001ULL;
+
 // one workgroup: the fixed-order sum of the partials (block_sum_partials'
-// order), then the value and the sequence number into host memory with
-// system-scope stores, the number last (release)
-__global__ __launch_bounds__(kB) void k_report(const double *__restrict__ part, uint64_t *slot,
-                                               uint64_t seq) {
+// order), then its bits into host memory with one system-scope store (a
+// single word carries the whole report, so no ordering is needed)
+__global__ __launch_bounds__(kB) void k_report(const double *__restrict__ part, uint64_t *slot) {
   __shared__ double sh[kB / 64];
   const double t = block_sum_partials(part, sh);
   if (threadIdx.x == 0) {
-    __hip_atomic_store(slot, __double_as_longlong(t), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(slot + 1, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    uint64_t bits = static_cast<uint64_t>(__double_as_longlong(t));
+    if (bits == kRepEmpty) bits ^= 1;  // keep the marker unambiguous
+    __hip_atomic_store(slot, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 }  // namespace
 
 uint64_t report_sum(Scratch &S, const double *part, hipStream_t s) {
   const uint64_t seq = ++S.rep_seq;
-  hipLaunchKernelGGL(k_report, dim3(1), dim3(kB), 0, s, part, S.rep_dev, seq);
+  // the previous report has been read (wait_report follows every
+  // report_sum), so the slot can be marked empty before this launch
+  __atomic_store_n(S.rep, kRepEmpty, __ATOMIC_SEQ_CST);
+  hipLaunchKernelGGL(k_report, dim3(1), dim3(kB), 0, s, part, S.rep_dev);
   LA_HIP(hipGetLastError());
   return seq;
 }
@@ -603,16 +609,17 @@ double wait_report(Scratch &S, uint64_t seq, hipStream_t s) {
   // spin on the coherent slot; after ~2 s without it, let the runtime
   // report a stream error (or the value, if it came meanwhile)
   const auto t0 = std::chrono::steady_clock::now();
+  (void)seq;
+  uint64_t bits;
   for (uint64_t spins = 0;; ++spins) {
-    if (__atomic_load_n(S.rep + 1, __ATOMIC_ACQUIRE) == seq) break;
+    if ((bits = __atomic_load_n(S.rep, __ATOMIC_ACQUIRE)) != kRepEmpty) break;
     if ((spins & 4095) == 4095 &&
         std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
       LA_HIP(hipStreamSynchronize(s));
-      if (__atomic_load_n(S.rep + 1, __ATOMIC_ACQUIRE) == seq) break;
+      if ((bits = __atomic_load_n(S.rep, __ATOMIC_ACQUIRE)) != kRepEmpty) break;
       throw la_error(LA_ERR_HIP, "device report did not arrive");
     }
   }
-  const uint64_t bits = __atomic_load_n(S.rep, __ATOMIC_RELAXED);
   double v;
   std::memcpy(&v, &bits, sizeof v);
   return v;

@@ -169,17 +169,19 @@ struct Scratch {
   DevBuf partials;  // kRedBlocks * kMaxDots doubles
   DevBuf scalars;   // device scalars used by the solvers
   double *host = nullptr;  // pinned mirror of a few scalars
-  // coherent pinned report slot {value, sequence number}, written by the
-  // device (report_sum) and polled by the host (wait_report)
+  // coherent pinned report slot: rep[0] holds the bits of the last sum, or
+  // an empty marker the host writes before each report_sum; written by the
+  // device and polled by the host (wait_report)
   uint64_t *rep = nullptr, *rep_dev = nullptr;
   uint64_t rep_seq = 0;
   Scratch();
   ~Scratch();
 };
-// the sum of kRedBlocks partials, stored with a new sequence number into the
-// report slot (no copy command, no event): returns the sequence number
+// the sum of kRedBlocks partials, stored as one word into the report slot
+// (no copy command, no event); returns a ticket for wait_report. Every
+// report_sum must be followed by its wait_report before the next one.
 uint64_t report_sum(Scratch &S, const double *part, hipStream_t s);
-// waits (host spin, bounded) until report `seq` has arrived; returns its value
+// waits (host spin, bounded) until the report has arrived; returns its value
 double wait_report(Scratch &S, uint64_t seq, hipStream_t s);
 constexpr int kMaxDots = 64;
 
