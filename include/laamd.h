@@ -329,7 +329,8 @@ typedef struct {
   double rel_tol;      /* mkl_iterative_solver::set_rel_tol (default 1e-8) */
   double abs_tol;      /* set_abs_tol (default 0)                          */
   int max_iters;       /* set_max_iters (default 1000)                     */
-  int restart;         /* FGMRES: set_restart_steps (default 30)           */
+  int restart;         /* FGMRES: set_restart_steps (default 30; 0 runs
+                          unrestarted GMRES up to max_iters, no upper cap) */
   int stop_rule;       /* la_stop_rule                                      */
   int precision;       /* la_precision                                     */
   int trsv_mode;       /* la_trsv_mode for the preconditioner apply (default SYNCFREE) */

@@ -138,9 +138,11 @@ struct SerialSPMV {
 };
 struct ParallelSPMV : SerialSPMV {};
 
-// SegSumSPMV / ALBUSSPMV (spmv.hpp:59-300): preprocess uploads the matrix
-// and its nnz-balanced segment table once; operator() reuses it. The thread
-// count of the reference has no meaning here (segments are fixed-size).
+// SegSumSPMV / ALBUSSPMV (spmv.hpp:59-300): preprocess uploads the pattern
+// and its nnz-balanced segment table once; operator() re-reads av on every
+// call, as the reference does (spmv.hpp:230-293), so values updated in place
+// between calls are used. The thread count of the reference has no meaning
+// here (segments are fixed-size).
 template <typename ROWTYPE = int, typename COLTYPE = int, typename VALTYPE = double,
           la_spmv_alg ALG = LA_SPMV_ALBUS>
 class BalancedSPMV {
@@ -151,27 +153,28 @@ class BalancedSPMV {
                   VALTYPE const *av) {
     A_ = std::make_shared<detail::MatHandle>(size, size, base, ai, aj, av);
     detail::check(la_spmv_preprocess(A_->h, ALG));
-    key_ = {ai, aj, av};
+    key_ = {ai, aj};
     size_ = size;
   }
   void operator()(COLTYPE size, int base, ROWTYPE const *ai, COLTYPE const *aj,
                   VALTYPE const *av, VALTYPE const *b, VALTYPE *x, VALTYPE alpha,
                   VALTYPE beta) const {
-    if (!A_ || key_ != Key{ai, aj, av} || size != size_) {
+    if (!A_ || key_ != Key{ai, aj} || size != size_) {
       detail::MatHandle A(size, size, base, ai, aj, av);
       detail::host_spmv(A.h, ALG, size, size, b, x, alpha, beta);
       return;
     }
+    detail::check(la_mat_set_values(A_->h, av));
     detail::host_spmv(A_->h, ALG, size, size, b, x, alpha, beta);
   }
 
  private:
   struct Key {
-    const void *a, *b, *c;
-    bool operator!=(const Key &o) const { return a != o.a || b != o.b || c != o.c; }
+    const void *a, *b;
+    bool operator!=(const Key &o) const { return a != o.a || b != o.b; }
   };
   std::shared_ptr<detail::MatHandle> A_;
-  Key key_{nullptr, nullptr, nullptr};
+  Key key_{nullptr, nullptr};
   COLTYPE size_ = 0;
 };
 template <typename R = int, typename C = int, typename V = double>
@@ -179,7 +182,11 @@ using SegSumSPMV = BalancedSPMV<R, C, V, LA_SPMV_SEGSUM>;
 template <typename R = int, typename C = int, typename V = double>
 using ALBUSSPMV = BalancedSPMV<R, C, V, LA_SPMV_ALBUS>;
 
-// SPMV<CSR, Impl> (spmv.hpp:310-334)
+// SPMV<CSR, Impl> (spmv.hpp:310-334). Like the reference, every product
+// reads the matrix's current values (`_matrix->AV()`): the pattern is
+// uploaded once, the values on each call. A caller whose values stay fixed
+// can say so with values_constant(true) to skip that upload in
+// apply_device (the device-resident form); the host form always uploads.
 template <typename CSRMatrixType, typename SPMVType> struct SPMV {
   using VALTYPE = typename CSRMatrixType::VALTYPE;
   void setMatrix(CSRMatrixType const *matrix) {
@@ -193,13 +200,17 @@ template <typename CSRMatrixType, typename SPMVType> struct SPMV {
   }
   void operator()(VALTYPE const *b, VALTYPE *x, VALTYPE alpha = 1., VALTYPE beta = 0.) const {
     if (!A_) const_cast<SPMV *>(this)->preprocess();
+    else detail::check(la_mat_set_values(A_->h, _matrix->AV()));
     detail::host_spmv(A_->h, alg(), _matrix->rows, _matrix->cols, b, x, alpha, beta);
   }
   // device-resident form: b and x are device pointers
   void apply_device(VALTYPE const *b, VALTYPE *x, VALTYPE alpha = 1., VALTYPE beta = 0.) const {
     if (!A_) const_cast<SPMV *>(this)->preprocess();
+    else if (!values_constant_) detail::check(la_mat_set_values(A_->h, _matrix->AV()));
     detail::check(la_spmv(A_->h, alg(), LA_F64, LA_F64, LA_F64, alpha, b, beta, x));
   }
+  // the caller promises not to change _matrix's values (apply_device only)
+  void values_constant(bool c) { values_constant_ = c; }
   static constexpr la_spmv_alg alg() {
     if constexpr (std::is_same_v<SPMVType, SerialSPMV> || std::is_same_v<SPMVType, ParallelSPMV>)
       return LA_SPMV_ROW;
@@ -211,6 +222,7 @@ template <typename CSRMatrixType, typename SPMVType> struct SPMV {
   CSRMatrixType const *_matrix = nullptr;
   std::shared_ptr<detail::MatHandle> A_;
   SPMVType _spmv;
+  bool values_constant_ = false;
 };
 
 // ------------------------------------------------------------- host helpers
@@ -812,7 +824,7 @@ class mkl_fgmres_solver : public mkl_iterative_solver {
   int solve_device_status(const double *b, double *x, la_solver_result *r) {
     la_prec_t P = detail_solver::prec_of(_P);
     la_solver_opts o = opts(P, _P.get());
-    o.restart = _num_restart > 0 ? _num_restart : std::min(_maxiter, 63);
+    o.restart = _num_restart;  // 0: unrestarted up to the iteration limit, as the reference
     if (_R)
       return la_fgmres_solve_right(detail_solver::device_matrix(_A), P,
                                    detail_solver::prec_of(_R), &o, b, x, r);

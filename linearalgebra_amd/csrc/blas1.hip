@@ -3,6 +3,7 @@
 // scalars, multi-dot / multi-axpy for GMRES orthogonalisation, precision
 // conversions.
 #include <chrono>
+#include <cmath>
 #include <cstring>
 
 #include "dev_util.hpp"
@@ -445,19 +446,20 @@ __global__ void k_f64_to_f32(int64_t n, const double *__restrict__ s, float *d) 
     d[t] = static_cast<float>(s[t]);
 }
 __device__ __forceinline__ uint16_t dev_f64_to_bf16(double v) {
-  // round the double directly at BF16 precision (7 stored mantissa bits):
-  // add half an ulp of bit 45 with ties-to-even, truncate, narrow exactly
+  // round the double directly to BF16 (8 significant bits), ties to even,
+  // with no intermediate rounding to FP32
   unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+  const uint16_t sign = static_cast<uint16_t>((u >> 48) & 0x8000);
   const unsigned long long exp = (u >> 52) & 0x7FF;
-  if (exp == 0x7FF) {  // inf / nan
-    const uint32_t f = __float_as_uint(static_cast<float>(v));
-    return static_cast<uint16_t>((f >> 16) | ((f & 0xFFFF) ? 0x40 : 0));
+  if (exp == 0x7FF) return sign | ((u << 12) ? 0x7FC0 : 0x7F80);  // nan / inf
+  if (exp < 1023 - 126) {
+    // below the smallest normal: BF16's subnormal grid is k * 2^-133; the
+    // scaling is exact and rint rounds half to even (k = 128 is 2^-126)
+    const double k = __builtin_rint(fabs(v) * 0x1p133);
+    return sign | static_cast<uint16_t>(k);
   }
-  if (exp < 1023 - 126 || exp > 1023 + 127) {  // outside float normals
-    const uint32_t f = __float_as_uint(static_cast<float>(v));
-    const uint32_t r = f + 0x7FFF + ((f >> 16) & 1);
-    return static_cast<uint16_t>(r >> 16);
-  }
+  if (exp > 1023 + 127) return sign | 0x7F80;  // |v| >= 2^128 overflows
+  // add half an ulp of bit 45 with ties to even, truncate, narrow exactly
   u += (1ULL << 44) - 1 + ((u >> 45) & 1);
   u &= ~((1ULL << 45) - 1);
   const uint32_t f = __float_as_uint(static_cast<float>(__longlong_as_double(static_cast<long long>(u))));
@@ -478,35 +480,6 @@ inline unsigned grid_for(int64_t n, int64_t cap = 4096) {
 }  // namespace
 
 // ------------------------------------------------------------------ host
-uint16_t double_to_bf16_rne(double d) {
-  uint64_t u;
-  std::memcpy(&u, &d, 8);
-  const uint64_t exp = (u >> 52) & 0x7FF;
-  auto via_float = [](double v) {
-    float f = static_cast<float>(v);
-    uint32_t b;
-    std::memcpy(&b, &f, 4);
-    if ((b & 0x7F800000U) == 0x7F800000U) return static_cast<uint16_t>((b >> 16) | ((b & 0xFFFF) ? 0x40 : 0));
-    return static_cast<uint16_t>((b + 0x7FFF + ((b >> 16) & 1)) >> 16);
-  };
-  if (exp == 0x7FF || exp < 1023 - 126 || exp > 1023 + 127) return via_float(d);
-  u += (1ULL << 44) - 1 + ((u >> 45) & 1);
-  u &= ~((1ULL << 45) - 1);
-  double r;
-  std::memcpy(&r, &u, 8);
-  float f = static_cast<float>(r);  // exact: r has at most 8 significant bits
-  uint32_t b;
-  std::memcpy(&b, &f, 4);
-  return static_cast<uint16_t>(b >> 16);
-}
-
-float bf16_to_float_host(uint16_t b) {
-  const uint32_t u = static_cast<uint32_t>(b) << 16;
-  float f;
-  std::memcpy(&f, &u, 4);
-  return f;
-}
-
 void launch_convert_f64(const double *src, int64_t n, la_dtype dst_dt, void *dst, hipStream_t s) {
   if (n == 0) return;
   if (dst_dt == LA_F32)
@@ -596,9 +569,12 @@ __global__ __launch_bounds__(kB) void k_report(const double *__restrict__ part, 
 }  // namespace
 
 uint64_t report_sum(Scratch &S, const double *part, hipStream_t s) {
+  // the slot is marked empty before the launch, which is safe only once the
+  // previous report's kernel has landed: a report that was never read (an
+  // error path left it) is drained first
+  if (S.rep_pending) LA_HIP(hipStreamSynchronize(s));
   const uint64_t seq = ++S.rep_seq;
-  // the previous report has been read (wait_report follows every
-  // report_sum), so the slot can be marked empty before this launch
+  S.rep_pending = seq;
   __atomic_store_n(S.rep, kRepEmpty, __ATOMIC_SEQ_CST);
   hipLaunchKernelGGL(k_report, dim3(1), dim3(kB), 0, s, part, S.rep_dev);
   LA_HIP(hipGetLastError());
@@ -608,8 +584,10 @@ uint64_t report_sum(Scratch &S, const double *part, hipStream_t s) {
 double wait_report(Scratch &S, uint64_t seq, hipStream_t s) {
   // spin on the coherent slot; after ~2 s without it, let the runtime
   // report a stream error (or the value, if it came meanwhile)
+  if (seq == 0 || seq != S.rep_pending)
+    throw la_error(LA_ERR_STATE, "wait_report: not the outstanding report");
+  S.rep_pending = 0;
   const auto t0 = std::chrono::steady_clock::now();
-  (void)seq;
   uint64_t bits;
   for (uint64_t spins = 0;; ++spins) {
     if ((bits = __atomic_load_n(S.rep, __ATOMIC_ACQUIRE)) != kRepEmpty) break;

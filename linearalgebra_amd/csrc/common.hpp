@@ -44,12 +44,6 @@ struct bf16 {
   uint16_t bits;
 };
 
-// Host: round a double to the nearest BF16, ties to even, in one step (no
-// double rounding through float).
-uint16_t double_to_bf16_rne(double d);
-float bf16_to_float_host(uint16_t b);
-// Host: double -> float, round to nearest even (the C conversion).
-inline float double_to_float(double d) { return static_cast<float>(d); }
 
 inline size_t dtype_size(la_dtype dt) {
   return dt == LA_F64 ? 8 : dt == LA_F32 ? 4 : 2;

@@ -174,12 +174,15 @@ struct Scratch {
   // device and polled by the host (wait_report)
   uint64_t *rep = nullptr, *rep_dev = nullptr;
   uint64_t rep_seq = 0;
+  uint64_t rep_pending = 0;  // ticket of the report not yet read, 0 if none
   Scratch();
   ~Scratch();
 };
 // the sum of kRedBlocks partials, stored as one word into the report slot
-// (no copy command, no event); returns a ticket for wait_report. Every
-// report_sum must be followed by its wait_report before the next one.
+// (no copy command, no event); returns a ticket for wait_report. One report
+// is outstanding at a time: a report_sum whose predecessor was never read
+// waits for the stream first; wait_report rejects any ticket but the
+// outstanding one.
 uint64_t report_sum(Scratch &S, const double *part, hipStream_t s);
 // waits (host spin, bounded) until the report has arrived; returns its value
 double wait_report(Scratch &S, uint64_t seq, hipStream_t s);

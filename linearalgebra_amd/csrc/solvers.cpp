@@ -38,7 +38,28 @@ struct Work {
   DevBuf v64[7], v32[7];
   DevBuf V, Z;  // GMRES bases
   la_dtype basis_dt = LA_F64;
+  // GMRES Hessenberg column (two Gram-Schmidt passes and the norm) on the
+  // device, and its pinned host staging; sized for the restart length
+  DevBuf hcol;
+  double *hpin = nullptr;
+  int hcap = 0;
   Scratch sc;
+  void ensure_hcol(int cap) {
+    if (cap <= hcap) return;
+    hcol.alloc(sizeof(double) * (2 * static_cast<size_t>(cap) + 1));
+    if (hpin) (void)hipHostFree(hpin);
+    hpin = nullptr;
+    LA_HIP(hipHostMalloc(reinterpret_cast<void **>(&hpin),
+                         sizeof(double) * (2 * static_cast<size_t>(cap) + 1),
+                         hipHostMallocDefault));
+    hcap = cap;
+  }
+  Work() = default;
+  Work(const Work &) = delete;
+  Work &operator=(const Work &) = delete;
+  ~Work() {
+    if (hpin) (void)hipHostFree(hpin);
+  }
 };
 
 thread_local std::unique_ptr<Work> tl_work;
@@ -178,20 +199,33 @@ CgResult fgmres_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_
     W.m = m;
     W.basis_dt = vec;
   }
-  LA_CHECK_ARG(m + 1 <= kMaxDots, "restart too large");
   char *V = static_cast<char *>(W.V.p), *Z = static_cast<char *>(W.Z.p);
   auto Vj = [&](int j) { return static_cast<void *>(V + es * static_cast<size_t>(n) * j); };
   auto Zj = [&](int j) { return static_cast<void *>(Z + es * static_cast<size_t>(n) * j); };
   DevBuf *vb = vec == LA_F64 ? W.v64 : W.v32;
   void *w = vb[0].p;
-  double *sc = W.sc.scalars.as<double>(), *part = W.sc.partials.as<double>();
-  double *h1 = sc + 8, *h2 = sc + 8 + kMaxDots, *hn = sc + 8 + 2 * kMaxDots;
+  double *part = W.sc.partials.as<double>();
+  // Hessenberg column: h1 and h2 hold the two passes' m + 1 coefficients,
+  // hn the norm; the update coefficients reuse h1
+  W.ensure_hcol(m + 1);
+  const int cap = m + 1;
+  double *h1 = W.hcol.as<double>(), *h2 = h1 + cap, *hn = h2 + cap;
   const la_spmv_alg alg = static_cast<la_spmv_alg>(o.spmv_alg);
   std::vector<double> H(static_cast<size_t>(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
   // pinned staging for the Hessenberg column and the update coefficients
   // (a pageable buffer would add a staged copy per iteration)
-  double *hbuf = W.sc.host + 16;
-  static_assert(16 + 2 * kMaxDots + 1 <= 2 * kRedBlocks, "pinned scratch too small");
+  double *hbuf = W.hpin;
+  // Gram-Schmidt against basis vectors [0, k): the dot products and the
+  // update in groups of at most kMaxDots vectors (one group up to m = 63)
+  auto gs_dots = [&](int k, double *out) {
+    for (int c0 = 0; c0 < k; c0 += kMaxDots)
+      multidot_dev(n, std::min(kMaxDots, k - c0), vec, Vj(c0), n, w, part, out + c0, s);
+  };
+  auto gs_update = [&](char *B, int k, const double *h, void *dst) {
+    for (int c0 = 0; c0 < k; c0 += kMaxDots)
+      multiaxpy_dev(n, std::min(kMaxDots, k - c0), vec, B + es * static_cast<size_t>(n) * c0, n,
+                    h + c0, dst, s);
+  };
   CgResult R;
   int total = 0;
   bool done = false;
@@ -210,17 +244,15 @@ CgResult fgmres_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_
     for (int j = 0; j < m; ++j) {
       apply_M(P, o, val, vec, Vj(j), Zj(j), n, s);
       op_apply(A, o, alg, val, vec, acc, 1.0, Zj(j), 0.0, w, w, W, s);
-      const int64_t ld = n;
-      multidot_dev(n, j + 1, vec, V, ld, w, part, h1, s);
-      multiaxpy_dev(n, j + 1, vec, V, ld, h1, w, s);
-      multidot_dev(n, j + 1, vec, V, ld, w, part, h2, s);
-      multiaxpy_dev(n, j + 1, vec, V, ld, h2, w, s);
+      gs_dots(j + 1, h1);
+      gs_update(V, j + 1, h1, w);
+      gs_dots(j + 1, h2);
+      gs_update(V, j + 1, h2, w);
       dot_dev(n, vec, w, w, part, hn, s);
-      LA_HIP(hipMemcpyAsync(hbuf, h1, sizeof(double) * (2 * kMaxDots + 1),
-                            hipMemcpyDeviceToHost, s));
+      LA_HIP(hipMemcpyAsync(hbuf, h1, sizeof(double) * (2 * cap + 1), hipMemcpyDeviceToHost, s));
       LA_HIP(hipStreamSynchronize(s));
-      for (int i = 0; i <= j; ++i) H[static_cast<size_t>(i) * m + j] = hbuf[i] + hbuf[kMaxDots + i];
-      const double hnorm = std::sqrt(hbuf[2 * kMaxDots]);
+      for (int i = 0; i <= j; ++i) H[static_cast<size_t>(i) * m + j] = hbuf[i] + hbuf[cap + i];
+      const double hnorm = std::sqrt(hbuf[2 * cap]);
       H[static_cast<size_t>(j + 1) * m + j] = hnorm;
       for (int i = 0; i < j; ++i) {
         const double a = H[static_cast<size_t>(i) * m + j], c = H[static_cast<size_t>(i + 1) * m + j];
@@ -253,7 +285,7 @@ CgResult fgmres_core(Mat &A, Prec *P, const la_solver_opts &o, la_dtype val, la_
     // x += Z y  (multiaxpy subtracts, so pass -y)
     for (int l = 0; l < k; ++l) hbuf[l] = -y[l];
     LA_HIP(hipMemcpyAsync(h1, hbuf, sizeof(double) * k, hipMemcpyHostToDevice, s));
-    multiaxpy_dev(n, k, vec, Z, n, h1, x, s);
+    gs_update(Z, k, h1, x);
     LA_HIP(hipStreamSynchronize(s));
   }
   R.iters = total;
@@ -449,7 +481,10 @@ int fgmres_solve(Mat &A, Prec *P, const la_solver_opts &o, const double *b, doub
   LA_CHECK_ARG(A.rows == A.cols, "FGMRES needs a square matrix");
   LA_CHECK_ARG(!P || P->n == A.rows, "preconditioner size mismatch");
   hipStream_t s = A.stream;
-  const int m = o.restart > 0 ? o.restart : std::min(o.max_iters, kMaxDots - 1);
+  // restart 0: unrestarted, up to max_iters (mkl_solver.cpp:267-269, 293-294)
+  LA_CHECK_ARG(o.restart >= 0 && o.max_iters >= 0, "negative restart or max_iters");
+  const int m = std::max(1, o.restart > 0 ? std::min(o.restart, std::max(o.max_iters, 1))
+                                          : o.max_iters);
   return run_solver(A, P, o, b, x, res,
                     [&](la_dtype val, la_dtype vec, const void *bb, void *xx, double nb,
                         double rt, double at, int mi, Work &W) {

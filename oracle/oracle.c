@@ -20,6 +20,11 @@
 double orc_wtime(void) { return omp_get_wtime(); }
 int orc_max_threads(void) { return omp_get_max_threads(); }
 
+/* Planted mistakes, for the tests that show their checks catch a wrong
+ * oracle: 0 = none (the default; nothing but those tests sets it). */
+static int g_plant = 0;
+void orc_set_plant(int which) { g_plant = which; }
+
 /* utils/utils.h:164-186: even split of `total` items over `nthreads`,
  * the first `total % nthreads` parts one item larger. */
 static void balanced_part(long total, int tid, int nthreads, long *s, long *e) {
@@ -58,7 +63,9 @@ void orc_spmv_serial(int n, int base, const int *ai, const int *aj,
                      double alpha, double beta) {
   for (int i = 0; i < n; i++) {
     double val = 0;
-    for (int j = ai[i] - base; j < ai[i + 1] - base; j++)
+    const int end = ai[i + 1] - base - (g_plant == ORC_PLANT_SPMV_DROP_LAST &&
+                                        ai[i + 1] > ai[i]);
+    for (int j = ai[i] - base; j < end; j++)
       val += av[j] * b[aj[j] - base];
     x[i] = alpha * val + (beta == 0 ? 0 : beta * x[i]);
   }
@@ -1079,6 +1086,7 @@ static void pseudo_diameter(int n, int base, const int *ai, const int *aj,
     backwardWidth = INT_MAX;
     for (int c = 0; c < nchosen; c++) {
       int i = chosen[c], h2 = 0, nl2 = 0;
+      if (g_plant == ORC_PLANT_RCM_WIDTH_RESET) width = 0;
       if (!pbfs_lastlevel(n, base, ai, aj, i, backwardWidth, &h2, &width,
                           last2, &nl2, visited, bufa, bufb))
         continue;
@@ -1117,9 +1125,10 @@ static int rem_find(int *par, int x) {
 void orc_rcm(int n, int base, const int *ai, const int *aj, int *iperm,
              int *perm) {
   int *deg = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+  int *deg0 = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
   int *par = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
   for (int i = 0; i < n; i++) {
-    deg[i] = ai[i + 1] - ai[i];
+    deg[i] = deg0[i] = ai[i + 1] - ai[i];
     par[i] = i;
   }
   for (int i = 0; i < n; i++)
@@ -1181,11 +1190,13 @@ void orc_rcm(int n, int base, const int *ai, const int *aj, int *iperm,
             levels[v] = -1;
           }
         }
-        /* insertion sort by (degree, index) (Reordering.cpp:117-122) */
+        /* insertion sort by (degree, index) (Reordering.cpp:117-122),
+         * with the degrees PseudoDiameter overwrote */
+        const int *sd = g_plant == ORC_PLANT_RCM_ORIGINAL_DEGREES ? deg0 : deg;
         for (int a = 1; a < nc; a++) {
           int v = children[a], t = a - 1;
-          while (t >= 0 && (deg[children[t]] > deg[v] ||
-                            (deg[children[t]] == deg[v] && children[t] > v))) {
+          while (t >= 0 && (sd[children[t]] > sd[v] ||
+                            (sd[children[t]] == sd[v] && children[t] > v))) {
             children[t + 1] = children[t];
             t--;
           }
@@ -1200,6 +1211,6 @@ void orc_rcm(int n, int base, const int *ai, const int *aj, int *iperm,
     int t = iperm[i]; iperm[i] = iperm[j]; iperm[j] = t;
   }
   for (int i = 0; i < n; i++) perm[iperm[i] - base] = i + base;
-  free(deg); free(par); free(root); free(cidx); free(cpre); free(fill);
+  free(deg); free(deg0); free(par); free(root); free(cidx); free(cpre); free(fill);
   free(sorted); free(levels); free(prefix); free(children); free(compOf);
 }

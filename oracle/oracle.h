@@ -187,15 +187,31 @@ void orc_dtad(int n, int base, const int *ai, const int *aj, double *av,
 
 /* ---- RCM (reordering/Reordering.cpp:37-136 and what it calls) ---- */
 /* iperm[new] = old + base, perm = inverse. The reference's BFS width member
- * is uninitialised (reordering/BFS.h:47); here it starts at 0. Exact
- * ordering unpinned (only a reference build could pin it); pinned: BFS
- * known-answer test (tests/bfs_test.cpp:12-51), permutation validity,
- * bandwidth reduction, base/run determinism. */
+ * is uninitialised (reordering/BFS.h:47); here it starts at 0 for every BFS
+ * object PseudoDiameter constructs and is never reset between the calls on
+ * one object, as there. Where the reference would loop forever (no
+ * candidate left in the last level, Reordering.h:91-130 with every degree
+ * overwritten), the search stops and keeps the current source.
+ * pinned: exact iperm of four small graphs derived by hand from
+ * Reordering.cpp:37-136, Reordering.h:73-136 and BFS.cpp:10-180 (the
+ * degree overwrite decides a child order; a second search round; the
+ * never-reset width decides the end point; components of sizes 1, 2, 4
+ * and 5), the BFS known-answer test (tests/bfs_test.cpp:12-51),
+ * permutation validity, bandwidth reduction, base/run determinism. */
 void orc_rcm(int n, int base, const int *ai, const int *aj, int *iperm,
              int *perm);
 /* Serial BFS levels (reordering/BFS.cpp:10-57); returns height. */
 int orc_bfs_levels(int n, int base, const int *ai, const int *aj, int source,
                    int *levels);
+
+/* Planted mistakes (tests of the tests only; 0 = none, the default). */
+enum {
+  ORC_PLANT_NONE = 0,
+  ORC_PLANT_SPMV_DROP_LAST = 1,       /* SerialSPMV skips each row's last entry */
+  ORC_PLANT_RCM_ORIGINAL_DEGREES = 2, /* RCM child sort by the unmodified degrees */
+  ORC_PLANT_RCM_WIDTH_RESET = 3       /* PseudoDiameter resets the width per BFS */
+};
+void orc_set_plant(int which);
 
 /* timing helper for the CPU baseline: seconds of wall clock */
 double orc_wtime(void);

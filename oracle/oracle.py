@@ -89,6 +89,7 @@ def _declare(L):
     L.orc_bfs_levels.restype = i
     L.orc_wtime.restype = d
     L.orc_max_threads.restype = i
+    L.orc_set_plant.argtypes = [i]
 
 
 def _i32(a):
@@ -360,6 +361,28 @@ def bfs_levels(A, source):
 
 def wtime():
     return lib().orc_wtime()
+
+
+# planted mistakes (oracle.h): only the tests that show their checks catch a
+# wrong oracle switch one on, always through `planted`
+PLANT_SPMV_DROP_LAST = 1
+PLANT_RCM_ORIGINAL_DEGREES = 2
+PLANT_RCM_WIDTH_RESET = 3
+
+
+class planted:
+    """Context manager: run the oracle with one planted mistake."""
+
+    def __init__(self, which):
+        self.which = which
+
+    def __enter__(self):
+        lib().orc_set_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_plant(0)
+        return False
 
 
 def max_threads():

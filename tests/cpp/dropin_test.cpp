@@ -64,6 +64,42 @@ int main() {
   for (int i = 0; i < n; ++i) ref2[i] = 2.0 * ref[i] - 1.0;
   expect(rel(y, ref2) < 1e-14, "SPMV<SegSum> alpha/beta", rel(y, ref2));
 
+  // values changed in place between calls (a refactor / Newton loop): every
+  // form reads the new av, as the reference's operator() does
+  {
+    std::vector<double> av2 = av;
+    for (size_t k = 0; k < av2.size(); ++k) av2[k] *= 1.0 + 0.25 * std::cos(0.3 * k);
+    std::vector<double> ref3(n);
+    for (int i = 0; i < n; ++i) {
+      double s = 0;
+      for (int k = ai[i]; k < ai[i + 1]; ++k) s += av2[k] * b[aj[k]];
+      ref3[i] = s;
+    }
+    std::vector<double> avm = av;  // the array the functors were given
+    matrix_utils::ALBUSSPMV<> a2(1);
+    a2.preprocess(n, 0, ai.data(), aj.data(), avm.data());
+    a2(n, 0, ai.data(), aj.data(), avm.data(), b.data(), x.data(), 1., 0.);
+    expect(rel(x, ref) < 1e-14, "ALBUSSPMV before the value change", rel(x, ref));
+    avm = av2;  // same pointer, new values
+    a2(n, 0, ai.data(), aj.data(), avm.data(), b.data(), x.data(), 1., 0.);
+    expect(rel(x, ref3) < 1e-14, "ALBUSSPMV reads values changed in place", rel(x, ref3));
+    matrix_utils::CSRMatrixVec<int, int, double> M2 = M;
+    matrix_utils::SPMV<decltype(M2), matrix_utils::ALBUSSPMV<>> s2;
+    s2.setMatrix(&M2);
+    s2.preprocess();
+    s2(b.data(), x.data());
+    expect(rel(x, ref) < 1e-14, "SPMV<ALBUS> before the value change", rel(x, ref));
+    M2.av = av2;
+    s2(b.data(), x.data());
+    expect(rel(x, ref3) < 1e-14, "SPMV<ALBUS> reads values changed in place", rel(x, ref3));
+    detail::DeviceBuffer db(sizeof(double) * n), dx(sizeof(double) * n);
+    db.upload(b.data(), sizeof(double) * n);
+    M2.av = av;
+    s2.apply_device(static_cast<const double *>(db.get()), static_cast<double *>(dx.get()));
+    dx.download(x.data(), sizeof(double) * n);
+    expect(rel(x, ref) < 1e-14, "SPMV<ALBUS>::apply_device reads the values again", rel(x, ref));
+  }
+
   // triangular solves (triangle_solve.hpp) on L/D/U of A
   matrix_utils::CSRMatrixVec<int, int, double> Lm, Um;
   std::vector<double> D;

@@ -73,3 +73,55 @@ def rel_err(x, ref):
     x, ref = np.asarray(x, np.float64), np.asarray(ref, np.float64)
     den = np.linalg.norm(ref)
     return np.linalg.norm(x - ref) / (den if den > 0 else 1.0)
+
+
+def bf16_rne(x):
+    """FP64 -> the nearest BF16 value (8 significant bits, FP32's exponent
+    range, subnormals k * 2^-133), ties to even, returned as float64; NaN
+    stays NaN and |x| at or past the halfway point above the largest BF16
+    becomes inf. Exact integer-valued arithmetic: x is scaled by the power
+    of two that puts its BF16 spacing at 1, rounded by rint (half to even)
+    and scaled back."""
+    x = np.asarray(x, np.float64)
+    out = np.array(x, copy=True)
+    fin = np.isfinite(x) & (x != 0)
+    xf = x[fin]
+    _, e = np.frexp(xf)               # |xf| = m 2^e, 0.5 <= m < 1
+    e = np.maximum(e, -125)           # the subnormal range keeps spacing 2^-133
+    spacing_exp = e - 8               # 8 significant bits
+    q = np.rint(np.ldexp(xf, -spacing_exp))
+    r = np.ldexp(q, spacing_exp)
+    r[np.abs(r) >= 2.0 ** 128] = np.inf * np.sign(r[np.abs(r) >= 2.0 ** 128])
+    out[fin] = r
+    return out
+
+
+def bf16_bits(x):
+    """BF16 bit patterns of values already on the BF16 grid (bf16_rne)."""
+    return (np.asarray(x, np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def bf16_special_values():
+    """Doubles that probe the FP64 -> BF16 rounding: exact ties (even and
+    odd neighbours), one FP64 ulp either side of a tie, the subnormal range
+    (ties at k + 1/2 times 2^-133, below half the smallest subnormal), the
+    largest finite values and the overflow threshold, signed zeros, inf."""
+    v = []
+    for m in (1.0, 1.5, 1.0078125, 1.9921875, 1.25):
+        for ex in (-126, -100, -3, 0, 7, 100, 127):
+            base = m * 2.0 ** ex
+            half = 2.0 ** (ex - 8)    # half a BF16 ulp in this binade
+            for t in (base + half, base - half):
+                v += [t, np.nextafter(t, np.inf), np.nextafter(t, -np.inf)]
+            v.append(base)
+    sub = 2.0 ** -133
+    for k in (0.5, 1.0, 1.5, 2.5, 3.5, 126.5, 127.5, 127.75, 0.25, 0.75):
+        t = k * sub
+        v += [t, np.nextafter(t, np.inf), np.nextafter(t, -np.inf)]
+    v += [2.0 ** -149, 2.0 ** -1074, 2.0 ** -126 * (1 - 2.0 ** -20)]
+    mx = (2 - 2.0 ** -7) * 2.0 ** 127  # largest BF16
+    thr = 2.0 ** 128 - 2.0 ** 119      # halfway to 2^128: rounds to even (inf)
+    v += [mx, np.nextafter(thr, 0), thr, np.nextafter(thr, np.inf), 2.0 ** 128, 1e300,
+          0.0, -0.0, np.inf]
+    v = np.array(v, np.float64)
+    return np.concatenate([v, -v])

@@ -229,3 +229,25 @@ def test_pcg_ex5_reference_fixture(dev):
     res = la.pcg(dA, P, b, x, rel_tol=1e-8, max_iters=100)
     xo, ok2, it, rr = orc.pcg(A, np.ones(A.n), kind="ic", F=R, rel_tol=1e-8, maxiter=100)
     assert res["converged"] and ok2 and res["iters"] == it
+
+
+@pytest.mark.parametrize("restart", [0, 100])
+def test_fgmres_long_restart(dev, restart):
+    """GMRES restart lengths above 63 and restart 0, which the reference
+    turns into unrestarted GMRES up to the iteration limit
+    (mkl_solver.cpp:267-269, 293-294): unpreconditioned 2-D Poisson 48^2
+    needs well over 64 Krylov vectors; the product's iteration count
+    follows the oracle's within 3 and both solutions agree."""
+    import torch
+    A = _o(la.gen.poisson2d(48))
+    b = la.gen.uniform(A.n, 31)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    x = torch.zeros(A.n, dtype=torch.float64, device=dev)
+    res = la.fgmres(dA, None, torch.tensor(b, device=dev), x, rel_tol=1e-10, max_iters=400,
+                    restart=restart)
+    xo, ok, it, rr = orc.fgmres(A, b, restart=restart, rel_tol=1e-10, maxiter=400)
+    print(res, "oracle", ok, it, rr, flush=True)
+    assert ok and it > 64
+    assert res["converged"] and res["rel_residual"] < 1.01e-10
+    assert abs(res["iters"] - it) <= 3
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-6 * np.linalg.norm(xo)

@@ -4,14 +4,20 @@ restatement (sparse_mat_op/spmv.hpp:12-29).
 Tolerances: FP64 values/vectors/accumulation -- norm-wise relative error
 <= 1e-12 and element-wise |y - ref| <= 1e-12 * (|alpha| |A| |x| + |beta| |y0|)_i
 (only summation order / FMA rounding may differ). FP32 / BF16 storage is
-checked against the FP64 oracle with the storage rounding bound:
-<= 2e-6 (FP32) and <= 2e-2 (BF16) of (|A| |x|)_i.
+checked twice: against the oracle run on the values rounded to the storage
+type by the tests' own round-to-nearest-even (only the FP32 accumulation
+and the FP32 vector may differ: <= 2e-6 of (|A| |x|)_i), and against the
+FP64 oracle with the storage rounding bound (unit roundoff 2^-24 / 2^-9,
+doubled, plus the accumulation term): <= 2e-6 (FP32) and <= 2^-8 + 2e-6
+(BF16) of (|A| |x|)_i. The device FP64 -> BF16 conversion is checked bit
+for bit on a diagonal matrix.
 """
 import numpy as np
 import pytest
 
 import linearalgebra_amd as la
-from helpers import as_lib_csr, from_dense, orc, random_sparse, read_mtx
+from helpers import (as_lib_csr, bf16_bits, bf16_rne, bf16_special_values, from_dense, orc,
+                     random_sparse, read_mtx)
 
 pytestmark = pytest.mark.gpu
 
@@ -129,8 +135,17 @@ def test_spmv_beta_zero_ignores_y(dev, alg):
     np.testing.assert_allclose(y.cpu().numpy(), orc.spmv(A, x), rtol=0, atol=1e-13)
 
 
+BF16_BOUND = 2.0 ** -8 + 2e-6   # twice BF16's unit roundoff + FP32 accumulation
+ACC32_BOUND = 2e-6              # FP32 accumulation and FP32 vectors
+
+
+def _round_values(A, val):
+    av = A.av.astype(np.float32).astype(np.float64) if val == la.F32 else bf16_rne(A.av)
+    return orc.CSR(A.n, A.ai, A.aj, av, ncols=A.ncols)
+
+
 @pytest.mark.parametrize("name,make", _cases()[:7], ids=[c[0] for c in _cases()[:7]])
-@pytest.mark.parametrize("val,bound", [(la.F32, 2e-6), (la.BF16, 2e-2)])
+@pytest.mark.parametrize("val,bound", [(la.F32, 2e-6), (la.BF16, BF16_BOUND)])
 @pytest.mark.parametrize("alg", [la.SPMV_ROW, la.SPMV_ALBUS, la.SPMV_SELL])
 def test_spmv_lowprec(dev, name, make, val, bound, alg):
     import torch
@@ -139,13 +154,46 @@ def test_spmv_lowprec(dev, name, make, val, bound, alg):
     x = la.gen.uniform(A.ncols, 7)
     dA = la.DeviceMatrix(as_lib_csr(A))
     ref = orc.spmv(A, x)
+    ref_rounded = orc.spmv(_round_values(A, val), x)
     scale = _absbound(A, x)
     for vec in (torch.float32, torch.float64):
         xt = torch.tensor(x, device=dev, dtype=vec)
         yt = torch.zeros(A.n, device=dev, dtype=vec)
         dA.spmv(xt, yt, alg=alg, val=val, acc=la.F32)
         got = yt.double().cpu().numpy()
+        assert np.all(np.abs(got - ref_rounded) <= ACC32_BOUND * scale + 1e-30), (name, vec)
         assert np.all(np.abs(got - ref) <= bound * scale + 1e-30), (name, vec)
+
+
+@pytest.mark.parametrize("alg", [la.SPMV_ROW, la.SPMV_ALBUS, la.SPMV_SELL])
+def test_bf16_conversion_bitwise(dev, alg):
+    """The device FP64 -> BF16 conversion (la_mat_add_dtype) rounds to
+    nearest even in one step: a diagonal matrix times x = 1 in FP32 reads
+    back bf16(a_ii) exactly; ties, near-ties, subnormals, overflow, inf and
+    NaN are compared with the tests' integer-valued rounding."""
+    import torch
+    v = bf16_special_values()
+    v = np.concatenate([v, [np.nan, -np.nan]])
+    n = v.size
+    A = orc.CSR(n, np.arange(n + 1), np.arange(n), v)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    dA.add_dtype(la.BF16)
+    y = torch.zeros(n, device=dev, dtype=torch.float32)
+    dA.spmv(torch.ones(n, device=dev, dtype=torch.float32), y, alg=alg, val=la.BF16,
+            acc=la.F32)
+    got = y.cpu().numpy()
+    want = bf16_rne(v)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    fin = np.isfinite(want)
+    gb = got.view(np.uint32)
+    assert np.all(gb[fin] & 0xFFFF == 0)
+    nz = fin & (want != 0)
+    assert np.array_equal((gb[nz] >> 16).astype(np.uint16), bf16_bits(want[nz])), \
+        v[nz][(gb[nz] >> 16).astype(np.uint16) != bf16_bits(want[nz])]
+    assert np.all(got[want == 0] == 0)
+    inf = np.isinf(want)
+    assert np.array_equal(got[inf], want[inf].astype(np.float32))
 
 
 def test_spmv_f64_storage_f32_accumulate(dev):

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import linearalgebra_amd as la
-from helpers import as_lib_csr, orc, read_mtx
+from helpers import as_lib_csr, bf16_rne, orc, read_mtx
 
 pytestmark = pytest.mark.gpu
 
@@ -179,17 +179,33 @@ def test_ilu_apply_transpose(dev, mode):
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("val,tol", [(la.F32, 1e-4), (la.BF16, 5e-2)])
 def test_lowprec_apply(dev, mode, val, tol):
+    """IC(0) apply with FP32 / BF16 factors, FP32 arithmetic. Against the
+    oracle's FP64 apply with the factors rounded by the tests themselves
+    (off-diagonal values, and the reciprocal diagonal the plans store):
+    only the FP32 arithmetic differs, 1e-5 relative. Against the unrounded
+    FP64 apply: the storage error, carried through both triangles."""
     import torch
     A = _o(la.gen.laplace3d(16))
     U = orc.split_triangle(A, True)
     R, ok = orc.icc(U, 0)
     b = la.gen.uniform(A.n, 6)
     ref = orc.prec_apply(A, "ic", R, b)
+    rnd = (lambda v: v.astype(np.float32).astype(np.float64)) if val == la.F32 else bf16_rne
+    rows = np.repeat(np.arange(R.n), np.diff(R.ai))
+    on_diag = rows == R.aj
+    av = rnd(R.av)
+    av[on_diag] = 1.0 / rnd(1.0 / R.av[on_diag])
+    ref_r = orc.prec_apply(A, "ic", orc.CSR(R.n, R.ai, R.aj, av), b)
     P = la.Preconditioner(as_lib_csr(R), "ic")
     for vdt in (torch.float32, torch.float64):
         z = torch.zeros(A.n, dtype=vdt, device=dev)
         P.apply(torch.tensor(b, device=dev, dtype=vdt), z, mode=mode, val=val)
-        err = np.linalg.norm(z.double().cpu().numpy() - ref) / np.linalg.norm(ref)
+        zh = z.double().cpu().numpy()
+        err = np.linalg.norm(zh - ref) / np.linalg.norm(ref)
+        err_r = np.linalg.norm(zh - ref_r) / np.linalg.norm(ref_r)
+        print("val", val, "vec", vdt, "err vs fp64 %.2e, vs rounded factors %.2e" % (err, err_r),
+              flush=True)
+        assert err_r < 1e-5, (val, vdt, err_r)
         assert err < tol, (val, vdt, err)
 
 

@@ -237,6 +237,19 @@ def _sym_graphs():
                                              + sp.eye(6000)))]
 
 
+@pytest.mark.parametrize("base", [0, 1])
+def test_rcm_hand_derived_orders(base):
+    """The product's RCM, serial and threaded, gives the hand-derived
+    SerialCM orders of the oracle's pins exactly."""
+    from test_oracle import RCM_PINS, rcm_graph
+    for name, (n, edges, expect) in RCM_PINS.items():
+        A = as_lib_csr(rcm_graph(n, edges, base))
+        for nt in (None, 1, 2, 8):
+            ip, p = la.rcm(A) if nt is None else la.rcm(A, nthreads=nt)
+            np.testing.assert_array_equal(ip, np.array(expect) + base, err_msg=name)
+            np.testing.assert_array_equal(p[np.array(expect)], np.arange(n) + base)
+
+
 @pytest.mark.parametrize("name,make", _sym_graphs(), ids=[g[0] for g in _sym_graphs()])
 def test_rcm_threads_bit_exact(name, make):
     """The threaded RCM equals the oracle's serial restatement for every

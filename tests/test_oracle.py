@@ -11,7 +11,8 @@ import scipy.linalg
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from helpers import from_dense, orc, random_sparse, read_mtx, rel_err, to_dense
+from helpers import (bf16_bits, bf16_rne, bf16_special_values, from_dense, orc, random_sparse,
+                     read_mtx, rel_err, to_dense)
 
 
 def _o(A):
@@ -497,13 +498,165 @@ def test_rcm_path_graph_closed_form():
     assert np.max(np.abs(rows - B.aj)) == 1
 
 
+def rcm_graph(n, edges, base=0):
+    """Symmetric pattern of an undirected graph, every diagonal stored (as
+    the reference's matrices have it), rows sorted."""
+    D = np.eye(n)
+    for a, b in edges:
+        D[a, b] = D[b, a] = 1.0
+    return from_dense(D, base=base)
+
+
+# Exact SerialCM orders (Reordering.cpp:37-136 with Reordering.h:73-136 and
+# BFS.cpp:10-180), each derived by hand step by step; deg() counts the row's
+# stored entries including its diagonal (PNodeDegree, Reordering.cpp:28-35).
+RCM_PINS = {
+    # A 5-cycle 0-1-2-4-3-0, every degree 3. Source: first minimum of the
+    # view, 0. BFS(0): {0} {1,3} {2,4}: height 3, width 2, last {2,4}.
+    # First five: 2 (tie 2/4 -> smaller index); 2 and its neighbours 1, 4
+    # get INT_MAX; {2,4} then holds only INT_MAX -> candidates [2].
+    # BFS(2): {2} {1,4} {0,3}: height 3 (not > 3), width 2 < INT_MAX ->
+    # target 2; 2 > 2 is false: no swap. CM from 0: children {1,3} sorted
+    # by the overwritten degrees (1: INT_MAX, 3: 3) -> 3, 1; then 3 -> 4,
+    # 1 -> 2: [0,3,1,4,2], reversed [2,4,1,3,0]. Sorting by the original
+    # degrees or by index gives [4,2,3,1,0].
+    "cycle5": (5, [(0, 1), (0, 3), (1, 2), (2, 4), (3, 4)], [2, 4, 1, 3, 0]),
+    # Hub 0 joined to 1..4, plus edge 1-2; degrees 5,3,3,2,2. Source 3.
+    # BFS(3): {3} {0} {1,2,4}: height 3, width 3. First five: 4 (degree 2)
+    # -> 4, 0 overwritten; then 1 (3, tie with 2) -> 0, 1, 2 overwritten:
+    # candidates [4, 1]. BFS(4): {4} {0} {1,2,3}: height 3, width stays 3
+    # -> target 4, backward width 3. BFS(1) with the same object: its
+    # levels {1} {0,2} {3,4} leave the width at 3 (never reset), 3 < 3 is
+    # false -> target stays 4; 3 > 3 false: no swap. CM from 3: 3 -> 0 ->
+    # children {1,2,4} all INT_MAX -> by index: [3,0,1,2,4], reversed
+    # [4,2,1,0,3]. With the width reset per BFS, BFS(1) would report
+    # width 2 and win, then 3 > 2 swaps the ends: [4,3,2,0,1]. With the
+    # original degrees the children sort to 4, 1, 2: [2,1,4,0,3].
+    "hub5": (5, [(0, 1), (0, 2), (0, 3), (0, 4), (1, 2)], [4, 2, 1, 0, 3]),
+    # Degrees 4,6,3,3,3,3,3,3. Source 2. Round 1: BFS(2): {2} {0,1}
+    # {5,3,4,7} {6}: height 4, width 4, last {6}. First five: 6 -> 6, 4, 7
+    # overwritten; candidates [6]. BFS(6): {6} {4,7} {1} {0,2,3} {5}:
+    # height 5 > 4 and width 4 < INT_MAX -> new source 6, no target yet.
+    # Round 2 (new BFS object): BFS(6) width 3, height 5, last {5}. First
+    # five: 5 -> 5, 0, 3 overwritten; candidates [5]. BFS(5): {5} {0,3}
+    # {1,2} {4,7} {6}: height 5 (not > 5), width 3 -> target 5; 3 > 3
+    # false. CM from 6: {4,7} (both INT_MAX, by index), 4 -> {1}, 1 ->
+    # {0,2,3} sorted by degree (2: 3; 0, 3: INT_MAX) -> 2, 0, 3; 0 -> {5}:
+    # [6,4,7,1,2,0,3,5], reversed [5,3,0,2,1,7,4,6]. Original degrees give
+    # 2, 3, 0 ([5,0,3,2,1,7,4,6]); index order 0, 2, 3 ([5,3,2,0,1,7,4,6]);
+    # a width reset per BFS makes round 2's backward width 2 < 3 and swaps
+    # the ends ([6,7,4,1,2,3,0,5]).
+    "two_rounds8": (8, [(0, 1), (0, 2), (0, 5), (1, 2), (1, 3), (1, 4), (1, 7), (3, 5),
+                        (4, 6), (6, 7)], [5, 3, 0, 2, 1, 7, 4, 6]),
+    # Four components; roots (largest member, UnionFind.cpp:82-129) 7, 8,
+    # 10, 11 give the order {7}, {0,2,5,8}, {3,10}, {1,4,6,9,11}
+    # (ComponentsStat, members ascending). {7}: copied. Path 8-0-5-2:
+    # source 2 (first degree-2 node), BFS(2) height 4, last {8}; first
+    # five: 8 -> 8, 0 overwritten; BFS(8) height 4, width 1 -> target 8;
+    # CM from 2: [2,5,0,8]. {3,10}: copied. The 5-cycle on 1,4,6,9,11 is
+    # "cycle5" relabelled in increasing order: [1,9,4,11,6]. Concatenated
+    # [7,2,5,0,8,3,10,1,9,4,11,6] and reversed.
+    "components12": (12, [(0, 8), (0, 5), (2, 5), (3, 10), (1, 4), (1, 9), (4, 6), (6, 11),
+                          (9, 11)], [6, 11, 4, 9, 1, 10, 3, 8, 0, 5, 2, 7]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(RCM_PINS))
+@pytest.mark.parametrize("base", [0, 1])
+def test_rcm_hand_derived_orders(name, base):
+    """The oracle's RCM equals the hand-derived SerialCM order exactly, in
+    base 0 and base 1 (iperm[new] = old + base, perm its inverse)."""
+    n, edges, expect = RCM_PINS[name]
+    ip, p = orc.rcm(rcm_graph(n, edges, base))
+    np.testing.assert_array_equal(ip, np.array(expect) + base)
+    np.testing.assert_array_equal(p[np.array(expect)], np.arange(n) + base)
+
+
 # ----------------------------------------------------------------- planted
 
+@pytest.mark.parametrize("plant,caught", [
+    (orc.PLANT_RCM_ORIGINAL_DEGREES, {"cycle5", "hub5", "two_rounds8", "components12"}),
+    (orc.PLANT_RCM_WIDTH_RESET, {"hub5", "two_rounds8"}),
+])
+def test_rcm_pins_catch_planted_mistakes(plant, caught):
+    """A child sort by the unmodified degrees, or a BFS width reset per call,
+    planted in the oracle, fails exactly the hand-derived orders it changes."""
+    failed = set()
+    with orc.planted(plant):
+        for name, (n, edges, expect) in RCM_PINS.items():
+            ip, _ = orc.rcm(rcm_graph(n, edges))
+            if ip.tolist() != expect:
+                failed.add(name)
+    assert failed == caught
+    for name, (n, edges, expect) in RCM_PINS.items():  # switched off again
+        assert orc.rcm(rcm_graph(n, edges))[0].tolist() == expect
+
+
 def test_checks_catch_a_planted_mistake():
-    """A dropped term in the oracle's SpMV (one nonzero zeroed in the
-    matrix handed to it) is caught by the scipy cross-check used above."""
+    """A dropped term planted in the oracle's SerialSPMV (each row's last
+    entry skipped) fails the scipy cross-check and the reference's known
+    answer used above."""
     A = read_mtx("rdist1.mtx")
     x = np.random.default_rng(0).uniform(-1, 1, A.n)
-    bad = orc.CSR(A.n, A.ai, A.aj, A.av.copy())
-    bad.av[1234] = 0.0
-    assert rel_err(orc.spmv(bad, x), A.to_scipy() @ x) > 1e-8
+    assert rel_err(orc.spmv(A, x), A.to_scipy() @ x) < 1e-14
+    with orc.planted(orc.PLANT_SPMV_DROP_LAST):
+        assert rel_err(orc.spmv(A, x), A.to_scipy() @ x) > 1e-3
+        K = from_dense([[1, 2, 3], [0, 4, 5], [0, 0, 6]])
+        assert orc.spmv(K, [1, 2, 3]).tolist() != [14, 23, 18]
+    assert rel_err(orc.spmv(A, x), A.to_scipy() @ x) < 1e-14
+
+
+# ----------------------------------------------------------------- BF16
+
+def _bf16_brute(x):
+    """Nearest BF16 by exact rational comparison against every finite
+    positive BF16 bit pattern; ties to the even pattern."""
+    import bisect
+    from fractions import Fraction
+    if not hasattr(_bf16_brute, "grid"):
+        vals = []
+        for b in range(0x7F80):
+            f = np.array([b << 16], np.uint32).view(np.float32)[0]
+            vals.append(Fraction(float(f)))
+        _bf16_brute.grid = vals
+    grid = _bf16_brute.grid
+    if np.isnan(x):
+        return np.nan
+    sgn = -1.0 if np.signbit(x) else 1.0
+    a = Fraction(abs(float(x))) if np.isfinite(x) else None
+    if a is None:
+        return sgn * np.inf
+    top = grid[-1] + (grid[-1] - grid[-2]) / 2   # halfway to 2^128
+    if a >= top:
+        return sgn * np.inf                       # the tie goes to even: inf
+    k = bisect.bisect_left(grid, a)
+    if k == len(grid):
+        return sgn * float(grid[-1])              # above the largest, below halfway
+    if grid[k] == a:
+        return sgn * float(grid[k])
+    lo, hi = k - 1, k
+    dlo, dhi = a - grid[lo], grid[hi] - a
+    pick = lo if (dlo < dhi or (dlo == dhi and lo % 2 == 0)) else hi
+    return sgn * float(grid[pick])
+
+
+def test_bf16_reference_rounding():
+    """The tests' FP64 -> BF16 reference (rint on the scaled value) equals
+    exact nearest-even rounding by brute force on ties, near-ties,
+    subnormals, overflow and signed zeros, and on random doubles; a
+    truncating conversion differs."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([bf16_special_values(),
+                        rng.standard_normal(300) * 10.0 ** rng.integers(-40, 38, 300),
+                        rng.standard_normal(100) * 2.0 ** -130])
+    got = bf16_rne(x)
+    want = np.array([_bf16_brute(v) for v in x])
+    same = (got == want) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), x[~same]
+    assert np.array_equal(np.signbit(got[~np.isnan(got)]), np.signbit(x[~np.isnan(got)]))
+    assert (bf16_bits(got[np.isfinite(got)]).astype(np.uint32) << 16).view(np.float32).tolist() \
+        == got[np.isfinite(got)].astype(np.float32).tolist()
+    with np.errstate(over="ignore"):
+        x32 = x[np.isfinite(x)].astype(np.float32)
+    trunc = ((x32.view(np.uint32) >> 16) << 16).view(np.float32)
+    assert (trunc.astype(np.float64) != want[np.isfinite(x)]).sum() > 50
