@@ -303,6 +303,15 @@ int la_prec_set_sweeps(la_prec_t P, int sweeps);
 /* ====================== fused BLAS-1 (FP64 accumulation) =============== */
 int la_dot(int64_t n, la_dtype vec, const void *x, const void *y,
            double *result_host, void *hip_stream);
+/* la_dot without any allocation, for callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc);
+ * no counterpart in the reference. The result goes to device memory
+ * (*result_dev) on the given stream and the call returns without waiting.
+ * The partial sums live in a fixed per-GPU device array of 512 doubles
+ * (4 KiB, held until the process ends), whatever n is; calls that overlap
+ * on one GPU share it and race. Bitwise equal to la_dot. */
+int la_dot_noalloc(int64_t n, la_dtype vec, const void *x, const void *y,
+                   double *result_dev, void *hip_stream);
 /* y += a*x; *result_host = <y, z> (z may be y) */
 int la_axpy_dot(int64_t n, la_dtype vec, double a, const void *x, void *y,
                 const void *z, double *result_host, void *hip_stream);

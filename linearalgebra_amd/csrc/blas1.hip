@@ -2,9 +2,11 @@
 // (deterministic two-stage reduction), fused CG updates with device-side
 // scalars, multi-dot / multi-axpy for GMRES orthogonalisation, precision
 // conversions.
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "dev_util.hpp"
 #include "device.hpp"
@@ -601,6 +603,45 @@ double wait_report(Scratch &S, uint64_t seq, hipStream_t s) {
   double v;
   std::memcpy(&v, &bits, sizeof v);
   return v;
+}
+
+// ---------------------------------------------------- allocation-free dot
+// The partial sums of dot_noalloc live in a fixed device array, one copy per
+// GPU for the life of the process: kRedBlocks doubles (4 KiB), the grid of
+// the reduction, whatever n is. Host code never passes `scratch` itself (in
+// host code the name is only a stand-in); it uses the per-device address
+// hipGetSymbolAddress returns, looked up once per device.
+constexpr int kNoAllocScratch = kRedBlocks;
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+namespace {
+double *scratch_ptr() {
+  static std::mutex mu;
+  static std::array<double *, 64> ptrs{};
+  int dev = 0;
+  LA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= static_cast<int>(ptrs.size()))
+    throw la_error(LA_ERR_ARG, "device index out of range for the fixed scratch table");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void *p = nullptr;
+    LA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw la_error(LA_ERR_HIP, "no device address for the fixed scratch");
+    ptrs[dev] = static_cast<double *>(p);
+  }
+  return ptrs[dev];
+}
+}  // namespace
+
+void dot_noalloc(int64_t n, la_dtype vec, const void *x, const void *y, double *out_dev,
+                 hipStream_t s) {
+  double *part = scratch_ptr();
+  static_assert(kNoAllocScratch >= kRedBlocks, "scratch holds one partial per block");
+  // every block writes its partial (0 when it has no elements), so nothing
+  // left by an earlier call is read: no clearing is needed
+  dot_partials(n, vec, x, y, part, s);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kB), 0, s, 1, kRedBlocks, part, out_dev);
+  LA_HIP(hipGetLastError());
 }
 
 void dot_dev(int64_t n, la_dtype vec, const void *x, const void *y, double *partials, double *out,

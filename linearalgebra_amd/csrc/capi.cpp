@@ -596,6 +596,15 @@ int la_dot(int64_t n, la_dtype vec, const void *x, const void *y, double *result
   LA_CATCH
 }
 
+int la_dot_noalloc(int64_t n, la_dtype vec, const void *x, const void *y, double *result_dev,
+                   void *stream) {
+  LA_TRY
+  LA_CHECK_ARG(n >= 0 && (vec == LA_F64 || vec == LA_F32) && result_dev, "bad argument");
+  LA_CHECK_ARG(n == 0 || (x && y), "null vector");
+  dot_noalloc(n, vec, x, y, result_dev, static_cast<hipStream_t>(stream));
+  LA_CATCH
+}
+
 int la_axpy_dot(int64_t n, la_dtype vec, double a, const void *x, void *y, const void *z,
                 double *result, void *stream) {
   LA_TRY

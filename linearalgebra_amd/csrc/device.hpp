@@ -191,6 +191,10 @@ constexpr int kMaxDots = 64;
 // out[0] = <x,y>
 void dot_dev(int64_t n, la_dtype vec, const void *x, const void *y, double *partials,
              double *out, hipStream_t s);
+// out_dev[0] = <x,y> with the partial sums in a fixed per-device array: no
+// allocation; calls that overlap on one GPU share that array and race
+void dot_noalloc(int64_t n, la_dtype vec, const void *x, const void *y, double *out_dev,
+                 hipStream_t s);
 // out[i] = <V_i, w> for i < k; V is k vectors of length n, stride ld
 void multidot_dev(int64_t n, int k, la_dtype vec, const void *V, int64_t ld, const void *w,
                   double *partials, double *out, hipStream_t s);
