@@ -21,17 +21,21 @@ int32 indices, RCM-reordered on the host.
     launch duration from one pair of HIP events around the K timed launches
     on the stream they run on (so gaps between launches count too).
   * cpu_baseline: the oracle's OpenMP SpMV (a restatement of the reference's
-    ParallelSPMV) on this host's cores, timed on the same matrix for a
-    bounded ~10 s sample.
+    ParallelSPMV) on this host's cores, the median of single SpMVs timed
+    over a bounded ~10 s sample after one warm-up, with the CPU model and
+    core count, also in the reference's 16 B/nnz convention.
+  * c4, c3, c5: the other configurations' records as extra keys of the same
+    line (27-point 160^3 ILU(1)-GMRES(30) FP64 vs mixed precision with FP64
+    refinement; power-law SpMV in FP64/FP32/BF16; the adjoint of a 512^2
+    diffusion solve).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4]
-       [--no-pcg] [--no-cpu] [--pmc-probe] [--trsv level|syncfree] [--c4-grid G]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1..c5]
+       [--no-pcg] [--no-cpu] [--no-extra] [--pmc-probe] [--trsv level|syncfree]
+       [--c4-grid G]
 
---config c1 (the 256^2 Poisson chain next to the CPU oracle), --config c3
-(power-law SpMV in FP64/FP32/BF16), --config c4 (27-point
-160^3: SpMV, ILU(1)-GMRES(30) FP64 vs mixed precision) and --config c5
-(the adjoint of a 512^2 diffusion solve) print their own JSON records; the
-default (c2) prints the bench line.
+--config c1 (the 256^2 Poisson chain next to the CPU oracle), c3, c4 and c5
+print only that configuration's record; the default (c2) prints the bench
+line with the c4, c3 and c5 records (leave them out with --no-extra).
 """
 import argparse
 import json
@@ -43,6 +47,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's thread placement: one thread per core,
+# packed; set before any OpenMP runtime starts
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 METRIC = ("SpMV GB/s (%HBM peak); PCG time to 1e-10: mixed FP32/BF16 vs FP64, "
           "1 GPU")
@@ -65,6 +73,23 @@ def log(*a):
 
 def spmv_bytes(n, nnz, s_v=8, s_x=8, s_y=8, beta_nonzero=False):
     return nnz * (s_v + 4) + (n + 1) * 4 + n * s_x + n * s_y * (2 if beta_nonzero else 1)
+
+
+def mixed_summary(sec, groups):
+    """Mixed precision against FP64 with the same preconditioner (same
+    triangular-solve mode and sweep count): per group, the FP64 time, the
+    fastest mixed variant and the ratio. `groups` maps a label to (the FP64
+    variant, the mixed variants)."""
+    out = {}
+    for label, (ref, mixed) in groups.items():
+        mixed = [k for k in mixed if k in sec]
+        if ref not in sec or not mixed:
+            continue
+        best = min(mixed, key=lambda k: sec[k])
+        out[label] = {"fp64_s": sec[ref], "best_mixed": best, "mixed_s": sec[best],
+                      "speedup": round(sec[ref] / sec[best], 3),
+                      "all": {k: round(sec[ref] / sec[k], 3) for k in mixed}}
+    return out
 
 
 def build_c2(la, grid=128, rcm=True):
@@ -101,37 +126,71 @@ def time_spmv(la, torch, mats, x, y, steps, warmup, alg, val, acc, stream):
     return [e0.elapsed_time(e1) / steps], wall_ms
 
 
-def cpu_baseline(A, seconds=10.0):
+def cpu_info():
+    """Model string, logical CPUs this process may run on, and the physical
+    cores among them (distinct (package, core) pairs in /proc/cpuinfo)."""
+    if _CPU_INFO:
+        return _CPU_INFO
+    model, cores, cur, ident = "unknown", set(), {}, {}
+    allowed = os.sched_getaffinity(0)
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "processor":
+                cur = {"cpu": int(v)}
+            elif k == "model name":
+                model = v
+            elif k in ("cpu family", "model", "stepping"):
+                ident[k.replace(" ", "_")] = v
+            elif k == "physical id":
+                cur["pkg"] = v
+            elif k == "core id" and cur.get("cpu") in allowed:
+                cores.add((cur.get("pkg"), v))
+    except OSError:
+        pass
+    _CPU_INFO.update({"model": model, "ident": ident, "logical_cpus": len(allowed),
+                      "physical_cores": len(cores) or None})
+    return _CPU_INFO
+
+
+# read at start-up: an OpenMP runtime with OMP_PROC_BIND binds the main thread
+# to one core once it starts, which would hide the process's CPU set
+_CPU_INFO = {}
+
+
+def cpu_baseline(A, seconds=10.0, min_runs=10):
+    """The oracle's OpenMP ParallelSPMV restatement on this host's cores:
+    one warm-up, then single SpMVs timed one by one until `seconds` have
+    passed (at least `min_runs`); the median run is reported."""
     from oracle import oracle as orc
     Ao = orc.CSR(A.n, A.ai, A.aj, A.av)
     x = np.random.default_rng(1).uniform(-1, 1, A.n)
     threads = orc.max_threads()
     orc.spmv(Ao, x, alg="parallel", nthreads=threads)  # warm-up
-    reps, t0 = 0, time.perf_counter()
-    while True:
+    times, t_start = [], time.perf_counter()
+    while len(times) < min_runs or time.perf_counter() - t_start < seconds:
+        t0 = time.perf_counter()
         orc.spmv(Ao, x, alg="parallel", nthreads=threads)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or reps >= 100000:
+        times.append(time.perf_counter() - t0)
+        if len(times) >= 100000:
             break
-    per = el / reps
+    med = float(np.median(times))
     B = spmv_bytes(A.n, A.nnz)
-    return {"value": round(B / per / 1e9, 3), "unit": "GB/s", "cores": threads,
+    ci = cpu_info()
+    return {"value": round(B / med / 1e9, 3), "unit": "GB/s", "cores": threads,
             "kind": "port",
-            "sample": "%d FP64 SpMVs (oracle ParallelSPMV restatement, OpenMP, %d threads) "
-                      "on the same RCM-ordered 128^3 7-point matrix, %.1f s; %.2f ms per "
-                      "SpMV" % (reps, threads, el, per * 1e3),
-            "cpu_model": _cpu_model()}
-
-
-def _cpu_model():
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
+            "GBs_16B_per_nnz": round(16.0 * A.nnz / med / 1e9, 3),
+            "median_ms": round(med * 1e3, 4),
+            "p10_p90_ms": [round(float(np.percentile(times, q)) * 1e3, 4) for q in (10, 90)],
+            "runs": len(times),
+            "sample": "%d single FP64 SpMVs after one warm-up (oracle ParallelSPMV "
+                      "restatement, OpenMP, %d threads, OMP_PROC_BIND=%s OMP_PLACES=%s) on the "
+                      "same RCM-ordered 128^3 7-point matrix, %.1f s; value = median" % (
+                          len(times), threads, os.environ.get("OMP_PROC_BIND"),
+                          os.environ.get("OMP_PLACES"), sum(times)),
+            "cpu_model": ci["model"], "cpu_ident": ci["ident"],
+            "logical_cpus": ci["logical_cpus"], "physical_cores": ci["physical_cores"]}
 
 
 def run_pcg(la, torch, A, dev, stream, trsv_mode, variants, cpu=False):
@@ -313,7 +372,7 @@ def bench_c1(la, torch, dev, stream, args):
     runs["cpu_oracle_fp64_serial"] = {"seconds": round(time.perf_counter() - t0, 4), "iters": it,
                                       "rel_residual": float("%.2e" % rr)}
     out["ic0_pcg"] = runs
-    print(json.dumps(out))
+    return out
 
 
 def bench_c3(la, torch, dev, stream, args):
@@ -331,17 +390,17 @@ def bench_c3(la, torch, dev, stream, args):
         big = torch.empty(1 << 27, dtype=torch.float64, device=dev)
         torch.empty_like(big).copy_(big)
         torch.cuda.synchronize()
-        print(json.dumps({"probe": "c3 spmv x4 + copy 1GiB", "nnz": A.nnz}))
-        return
+        return {"probe": "c3 spmv x4 + copy 1GiB", "nnz": A.nnz}
     rl = np.diff(A.ai)
     mats = [la.DeviceMatrix(A, stream=stream) for _ in range(2)]
-    tab = _spmv_table(la, torch, A, mats, dev, stream,
-                      [("auto", la.SPMV_AUTO), ("albus", la.SPMV_ALBUS), ("row", la.SPMV_ROW)],
-                      [(la.F64, "f64"), (la.F32, "f32"), (la.BF16, "bf16")], args.steps)
-    print(json.dumps({"config": "c3 power-law n=4M", "n": A.n, "nnz": A.nnz,
-                      "auto_runs": ALG_NAMES[mats[0].resolve(la.SPMV_AUTO)],
-                      "max_row": int(rl.max()), "gen_s": prep, "spmv": tab,
-                      "note": "f32/bf16: values in that type, FP32 vectors and accumulation"}))
+    algs = [("auto", la.SPMV_AUTO), ("albus", la.SPMV_ALBUS), ("row", la.SPMV_ROW)]
+    tab = _spmv_table(la, torch, A, mats, dev, stream, algs,
+                      [(la.F64, "f64"), (la.F32, "f32"), (la.BF16, "bf16")], 20)
+    return {"config": "c3 power-law n=4M", "n": A.n, "nnz": A.nnz,
+            "auto_runs": ALG_NAMES[mats[0].resolve(la.SPMV_AUTO)],
+            "max_row": int(rl.max()), "gen_s": prep, "spmv": tab,
+            "note": "f32/bf16: values in that type, FP32 vectors and accumulation; GBs of "
+                    "the CSR algorithmic bytes for that value type"}
 
 
 def bench_c4(la, torch, dev, stream, args):
@@ -444,12 +503,12 @@ def bench_c4(la, torch, dev, stream, args):
                                        "bitwise_equal_host": same}},
            "trsv_mode": args.trsv, "gmres30": runs}
     sec = {k: v["seconds"] for k, v in runs.items()}
-    out["speedup_vs_plain_fp64"] = {k: round(sec["fp64"] / sec[k], 3)
-                                    for k in sec if k.startswith("mixed")}
-    out["speedup_vs_fp64_same_trsv"] = {
-        k: round(sec["fp64_jacobi6" if "jacobi" in k else "fp64"] / sec[k], 3)
-        for k in sec if k.startswith("mixed")}
-    print(json.dumps(out))
+    out["mixed_vs_fp64"] = mixed_summary(
+        sec, {"exact (%s)" % args.trsv: ("fp64", [k for k in sec if k.startswith("mixed")
+                                                   and "jacobi" not in k]),
+              "6 Jacobi sweeps": ("fp64_jacobi6", [k for k in sec if k.startswith("mixed")
+                                                   and "jacobi" in k])})
+    return out
 
 
 def bench_c5(la, torch, dev, stream, args):
@@ -529,10 +588,13 @@ def bench_c5(la, torch, dev, stream, args):
         log("c5", name, runs[name])
     sec = {k: v["seconds_two_forward_one_adjoint"] for k, v in runs.items()
            if "seconds_two_forward_one_adjoint" in v}
-    print(json.dumps({"config": "c5 adjoint 2-D diffusion 512^2", "n": nx * nx,
-                      "edges": ne, "cpu_reference_s": round(t_ref, 2), "runs": runs,
-                      "speedup_vs_plain_fp64": {k: round(sec["fp64"] / sec[k], 3)
-                                                for k in sec if k != "fp64"}}))
+    return {"config": "c5 adjoint 2-D diffusion 512^2", "n": nx * nx,
+            "edges": ne, "cpu_reference_s": round(t_ref, 2), "runs": runs,
+            "mixed_vs_fp64": mixed_summary(
+                sec, {"exact (syncfree)": ("fp64", [k for k in sec if k.startswith("mixed")
+                                                     and "jacobi" not in k]),
+                      "3 Jacobi sweeps": ("fp64_jacobi3", [k for k in sec if k.startswith("mixed")
+                                                           and "jacobi" in k])})}
 
 
 def main():
@@ -548,7 +610,10 @@ def main():
     ap.add_argument("--trsv", default="syncfree", choices=["level", "syncfree"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--c4-grid", type=int, default=160)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="leave out the c4, c3 and c5 records of the default line")
     args = ap.parse_args()
+    cpu_info()  # before torch or the libraries start an OpenMP runtime
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only; the multi-GPU sharding of the "
               "path is a design note (row-block halo exchange), not a measured mode.",
@@ -561,14 +626,10 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
 
-    if args.config == "c1":
-        return bench_c1(la, torch, dev, stream, args)
-    if args.config == "c3":
-        return bench_c3(la, torch, dev, stream, args)
-    if args.config == "c4":
-        return bench_c4(la, torch, dev, stream, args)
-    if args.config == "c5":
-        return bench_c5(la, torch, dev, stream, args)
+    others = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}
+    if args.config in others:
+        print(json.dumps(others[args.config](la, torch, dev, stream, args)))
+        return
     if args.config != "c2":
         raise SystemExit("--config is one of c2 (the bench line), c1, c3, c4, c5")
     A, prep = build_c2(la)
@@ -621,6 +682,17 @@ def main():
             "GBs": round(Bl / (np.mean(d2) * 1e-3) / 1e9, 1),
             "us": round(float(np.mean(d2)) * 1e3, 2)}
 
+    # the bytes the timed kernel's own layout streams (for sliced ELL: every
+    # slot's value and column, padding included, the slice table, x once
+    # and y), next to the CSR figure `value` is quoted in
+    layout = None
+    if chosen == 4 and sinfo["built"]:
+        nsl = (n + 63) // 64
+        Bl = sinfo["slots"] * (8 + (2 if sinfo["col16"] else 4)) + nsl * (8 + 4) + n * 8 + n * 8
+        layout = {"name": "sliced ELL, %s columns" % ("16-bit offset" if sinfo["col16"]
+                                                      else "int32"),
+                  "bytes_per_spmv": int(Bl), "GBs": round(Bl / (avg_ms * 1e-3) / 1e9, 1),
+                  "frac": round(Bl / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     traffic = None
     tpath = os.path.join(ROOT, "profiles",
                          "traffic_spmv_%s_f64_laplace3d_128.json" % ALG_NAMES[chosen])
@@ -651,23 +723,41 @@ def main():
         pcg = run_pcg(la, torch, A, dev, stream, mode, variants, cpu=not args.no_cpu)
         r = pcg["runs"]
         sec = {k: v["seconds"] for k, v in r.items()}
-        mixed = [k for k in sec if k.startswith("mixed")]
-        pcg["best_mixed"] = min(mixed, key=lambda k: sec[k])
-        # against the plain FP64 solve (the reference's algorithm) ...
-        pcg["speedup_vs_plain_fp64"] = {k: round(sec["fp64"] / sec[k], 3) for k in mixed}
-
-        # ... and against the fastest FP64 solve with the same triangular
-        # solves (same sweep count; either stop rule)
-        def same64(k):
-            tag = k.split("_", 2)[2] if k.count("_") >= 2 else ""
-            if not tag:
-                return min(sec["fp64"], sec["fp64_recurrence"])
-            return min(sec["fp64_" + tag], sec["fp64_%s_recurrence" % tag])
-        pcg["speedup_vs_fp64_same_trsv"] = {k: round(same64(k) / sec[k], 3) for k in mixed}
+        # the headline: mixed precision against FP64 with the same
+        # preconditioner (same triangular solves), FP64 at its faster stop
+        # rule; the exact IC(0) apply is the reference's preconditioner
+        groups = {}
+        for tag, label in [("", "exact IC(0) (%s)" % args.trsv), ("_jacobi3", "3 Jacobi sweeps"),
+                           ("_jacobi1", "1 Jacobi sweep")]:
+            f = "fp64" + tag
+            fr = "fp64%s_recurrence" % tag
+            ref = f if sec[f] <= sec[fr] else fr
+            groups[label] = (ref, ["mixed_f32" + tag, "mixed_bf16" + tag])
+        pcg["mixed_vs_fp64_same_preconditioner"] = mixed_summary(sec, groups)
+        # for the record only: the Jacobi variants change the preconditioner
+        # (and the iteration count), so these are not precision speedups
+        pcg["ratio_to_plain_fp64_exact_ic0"] = {k: round(sec["fp64"] / sec[k], 3)
+                                                for k in sec if k.startswith("mixed")}
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(A, args.cpu_seconds)
+
+    # the other configurations' records, each on its own (c2's matrices and
+    # plans freed first)
+    sub = {}
+    if not args.no_extra:
+        del mats
+        import gc
+        gc.collect()
+        torch.cuda.empty_cache()
+        for name, fn in [("c4", bench_c4), ("c3", bench_c3), ("c5", bench_c5)]:
+            t0 = time.time()
+            sub[name] = fn(la, torch, dev, stream, args)
+            sub[name]["record_wall_s"] = round(time.time() - t0, 1)
+            gc.collect()
+            torch.cuda.empty_cache()
+            log("record", name, "done in %.1f s" % sub[name]["record_wall_s"])
 
     line = {
         "metric": METRIC,
@@ -695,10 +785,22 @@ def main():
                      "traffic": traffic,
                      "kernel": (ALG_KERNEL[chosen] % ("short" if sinfo["col16"] else "int")
                                 if chosen == 4 else ALG_KERNEL[chosen]),
-                     "avg_launch_us": round(avg_ms * 1e3, 3)},
+                     "avg_launch_us": round(avg_ms * 1e3, 3),
+                     "achieved_basis": "CSR algorithmic bytes (nnz*(8+4) + (n+1)*4 + n*8 + "
+                                       "n*8) over the measured launch duration",
+                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this "
+                                        "kernel on this matrix, scaled by the factors that "
+                                        "make a 1 GiB device copy read and write 1 GiB "
+                                        "(committed counter summary %s, not measured by this "
+                                        "run)" % os.path.relpath(tpath, ROOT)) if traffic else None,
+                     "traffic_GBs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                     if traffic else None,
+                     "layout": layout},
         "cpu_baseline": cpu,
         "pcg": pcg,
     }
+    line.update(sub)
     print(json.dumps(line))
 
 

@@ -167,3 +167,49 @@ def test_c2_jacobi_apply_and_mixed_pcg_full(dev):
     bh = b.cpu().numpy()
     rr = bh - orc.spmv(_o(A), x.cpu().numpy(), alg="parallel", nthreads=orc.max_threads())
     assert np.linalg.norm(rr) <= 1.01e-10 * np.linalg.norm(bh)
+
+
+def test_c4_fgmres_full(dev):
+    """The 160^3 27-point solve (mkl_solver.cpp:253-386 with
+    incomplete_lu_k at level 1): ILU(1)-FGMRES(30) to a 1e-10 relative
+    residual in plain FP64, and mixed precision (FP32 matrix + FP32
+    factors; BF16 matrix + FP32 factors) with FP64 iterative refinement,
+    each with the exact sync-free triangular solves and with 6 Jacobi
+    sweeps per triangle. Every final true residual is recomputed on the
+    host with the oracle's ParallelSPMV (<= 1.01e-10 ||b||); every mixed
+    solution is compared with the FP64 solution with the same triangular
+    solves (both stop at a 1e-10 residual of this anisotropic operator)."""
+    import torch
+    A = la.gen.aniso27(160)
+    P = la.Preconditioner.ilu_gpu(A, 1)
+    P.refactor(torch.tensor(A.av, device=dev))
+    P.set_sweeps(6)
+    dA = la.DeviceMatrix(A)
+    bh = la.gen.uniform(A.n, 4)
+    b = torch.tensor(bh, device=dev)
+    Ao = _o(A)
+    J = dict(trsv_mode=la.TRSV_JACOBI)
+    S = dict(trsv_mode=la.TRSV_SYNCFREE)
+    variants = [("fp64", dict(precision=la.PREC_FP64, **S)),
+                ("mixed_f32", dict(precision=la.PREC_MIXED_F32, inner_tol=1e-5, **S)),
+                ("mixed_bf16_matrix_f32_factors",
+                 dict(precision=la.PREC_MIXED_BF16_F32P, inner_tol=1e-3, **S)),
+                ("fp64_jacobi6", dict(precision=la.PREC_FP64, **J)),
+                ("mixed_f32_jacobi6", dict(precision=la.PREC_MIXED_F32, inner_tol=1e-4, **J)),
+                ("mixed_bf16_matrix_f32_factors_jacobi6",
+                 dict(precision=la.PREC_MIXED_BF16_F32P, inner_tol=1e-3, **J))]
+    xs = {}
+    for name, kw in variants:
+        x = torch.zeros(A.n, dtype=torch.float64, device=dev)
+        res = la.fgmres(dA, P, b, x, rel_tol=1e-10, restart=30, max_iters=3000, **kw)
+        xh = x.cpu().numpy()
+        rr = bh - orc.spmv(Ao, xh, alg="parallel", nthreads=orc.max_threads())
+        true_rel = np.linalg.norm(rr) / np.linalg.norm(bh)
+        ref = xs.get("fp64_jacobi6" if "jacobi" in name else "fp64", xh)
+        dx = np.linalg.norm(xh - ref) / np.linalg.norm(ref)
+        print("c4 gmres", name, res, "host true residual %.3e, x vs fp64 %.3e" % (true_rel, dx),
+              flush=True)
+        assert res["converged"] and res["rel_residual"] < 1e-10
+        assert true_rel <= 1.01e-10
+        assert dx <= 1e-6
+        xs[name] = xh
