@@ -239,6 +239,12 @@ int la_trsv_destroy(la_trsv_t T);
 int la_trsv_add_dtype(la_trsv_t T, la_dtype dt);
 int la_trsv_set_stream(la_trsv_t T, void *hip_stream);
 int la_trsv_info(la_trsv_t T, int *n, int *nlevels, int64_t *nnz);
+/* Which schedule the sync-free mode runs (no counterpart in the reference):
+ * *tiled = 1 when the triangle's pattern was recognised as a grid and cut
+ * into *tiles tiles of edge *tile_edge grid steps, one workgroup each (the
+ * environment variable LAAMD_SF_TILED=0 at plan creation turns it off);
+ * 0 for the level-chunk schedule. */
+int la_trsv_schedule_info(la_trsv_t T, int *tiled, int *tiles, int *tile_edge);
 /* Sweep count of LA_TRSV_JACOBI solves with this plan (0 = diagonal scaling
  * only); no counterpart in the reference. */
 int la_trsv_set_sweeps(la_trsv_t T, int sweeps);
@@ -296,6 +302,9 @@ int la_prec_apply(la_prec_t P, la_trsv_mode mode, la_dtype val, la_dtype vec,
 int la_prec_apply_transpose(la_prec_t P, la_trsv_mode mode, la_dtype val,
                             la_dtype vec, const void *r, void *z);
 int la_prec_info(la_prec_t P, int *levels_lower, int *levels_upper);
+/* the tile count of each triangle's tiled sync-free schedule, 0 where the
+ * level-chunk schedule runs (no counterpart in the reference) */
+int la_prec_schedule_info(la_prec_t P, int *tiled_lower, int *tiled_upper);
 /* Sweep count of LA_TRSV_JACOBI applies, both triangles (default 3); no
  * counterpart in the reference. */
 int la_prec_set_sweeps(la_prec_t P, int sweeps);

@@ -196,7 +196,10 @@ class TriangularSolve:
     def info(self):
         n, lv, nnz = C.c_int(), C.c_int(), C.c_int64()
         check(lib().la_trsv_info(self.h, C.byref(n), C.byref(lv), C.byref(nnz)))
-        return {"n": n.value, "levels": lv.value, "nnz": nnz.value}
+        t, parts, edge = C.c_int(), C.c_int(), C.c_int()
+        check(lib().la_trsv_schedule_info(self.h, C.byref(t), C.byref(parts), C.byref(edge)))
+        return {"n": n.value, "levels": lv.value, "nnz": nnz.value, "tiled": bool(t.value),
+                "tiles": parts.value, "tile_edge": edge.value}
 
     def add_dtype(self, dt):
         check(lib().la_trsv_add_dtype(self.h, dt))
@@ -287,6 +290,13 @@ class Preconditioner:
     def levels(self):
         a, b = C.c_int(), C.c_int()
         check(lib().la_prec_info(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def tiles(self):
+        """Tile counts of the lower and upper solve's tiled sync-free
+        schedule (0: the level-chunk schedule)."""
+        a, b = C.c_int(), C.c_int()
+        check(lib().la_prec_schedule_info(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
     def add_dtype(self, dt):

@@ -109,6 +109,8 @@ _SIGS = {
     "la_prec_apply_transpose": [_vp, _i, _i, _i, _vp, _vp],
     "la_prec_info": [_vp, C.POINTER(_i), C.POINTER(_i)],
     "la_prec_set_sweeps": [_vp, _i],
+    "la_trsv_schedule_info": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)],
+    "la_prec_schedule_info": [_vp, C.POINTER(_i), C.POINTER(_i)],
     "la_dot": [_i64, _i, _vp, _vp, C.POINTER(_d), _vp],
     "la_dot_noalloc": [_i64, _i, _vp, _vp, _vp, _vp],
     "la_axpy_dot": [_i64, _i, _d, _vp, _vp, _vp, C.POINTER(_d), _vp],

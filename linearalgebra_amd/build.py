@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblaamd.so")
 
-SOURCES = ["host_algos.cpp", "rcm.cpp", "gen.cpp", "spmv.hip", "sptrsv.hip", "numeric.hip", "scale.hip", "blas1.hip",
+SOURCES = ["host_algos.cpp", "rcm.cpp", "tiling.cpp", "gen.cpp", "spmv.hip", "sptrsv.hip", "numeric.hip", "scale.hip", "blas1.hip",
            "solvers.cpp", "io.cpp", "capi.cpp"]
 
 

@@ -565,6 +565,21 @@ int la_prec_info(la_prec_t P, int *levels_lower, int *levels_upper) {
   if (levels_upper) *levels_upper = P->p->upper->nlevels;
   LA_CATCH
 }
+int la_trsv_schedule_info(la_trsv_t T, int *tiled, int *tiles, int *tile_edge) {
+  LA_TRY
+  LA_CHECK_ARG(T, "null handle");
+  if (tiled) *tiled = T->t->tiled ? 1 : 0;
+  if (tiles) *tiles = T->t->t_nparts;
+  if (tile_edge) *tile_edge = T->t->t_tau;
+  LA_CATCH
+}
+int la_prec_schedule_info(la_prec_t P, int *tiled_lower, int *tiled_upper) {
+  LA_TRY
+  LA_CHECK_ARG(P, "null handle");
+  if (tiled_lower) *tiled_lower = P->p->lower->tiled ? P->p->lower->t_nparts : 0;
+  if (tiled_upper) *tiled_upper = P->p->upper->tiled ? P->p->upper->t_nparts : 0;
+  LA_CATCH
+}
 int la_prec_set_sweeps(la_prec_t P, int sweeps) {
   LA_TRY
   LA_CHECK_ARG(P, "null handle");

@@ -3,6 +3,7 @@
 #pragma once
 #include <map>
 #include <memory>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -94,6 +95,19 @@ struct TrsvPlan {
   const double *fsrc = nullptr;
   // captured per-level launch sequences, keyed by (val, vec, b, x)
   std::map<std::tuple<int, int, const void *, void *>, hipGraphExec_t> graphs;
+  // tiled sync-free schedule (built when the pattern is recognised as a
+  // grid): one workgroup per tile, its rows' dependencies inside the tile
+  // kept in LDS; layout arrays on the device, values in tiled order per
+  // precision (refreshed with the level-ordered values)
+  bool tiled = false;
+  int t_nparts = 0, t_ngroups = 0, t_mmax = 0, t_hmax = 0, t_maxe = 0, t_tau = 0;
+  double t_remote = 0;
+  DevBuf t_gptr, t_grow, t_perm, t_code, t_rfptr, t_rfcol, t_rfslot;
+  DevBuf t_lsrc, t_dpos;  // tiled entry -> level-ordered entry (-1: padding); row -> level position
+  DevBuf t_bt;            // the right-hand side in tiled order (per solve)
+  MultiValues t_val, t_dinv;
+  unsigned t_wgs = 0;   // workgroups launched
+  std::string t_why;    // why the tiled schedule was not built
   // JACOBI mode (built on first use): x = D^-1 w_k with w_0 = b and
   // w_{s+1} = b - (N D^-1) w_s, N the strict triangle in row order
   int sweeps = 3;

@@ -1,6 +1,7 @@
 // Host-side (CPU) algorithms of the hot path; see host_algos.cpp.
 #pragma once
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace laamd {
@@ -44,6 +45,36 @@ bool icc_numeric(int n, int base, const int *ai, const int *aj, const double *av
 bool ick_numeric(int n, int base, const int *ai, const int *aj, const double *av,
                  const int *diag_pos, double shift, const int *icc_ai,
                  const int *icc_aj, double *icc_av);
+
+// Tiled schedule of the sync-free triangular solve (tiling.cpp): rows cut
+// into tiles of a grid the pattern is recognised as, each tile's rows in
+// (level, index) order in groups of at most `lanes` rows of one level.
+constexpr int kTileMaxEntries = 8;  // longest row the tiled kernel takes
+constexpr int kTileWindow = 16;     // groups a tile keeps its results for
+struct TileLayout {
+  int nparts = 0, ngroups = 0;
+  int m_max = 0;   // rows of the largest tile
+  int h_max = 0;   // most values one tile reads from other tiles
+  int maxe = 0;    // entries per row slot (4 or kTileMaxEntries)
+  int tau = 0;     // tile edge in grid steps
+  double remote_frac = 0;  // entries read from another tile
+  std::vector<int> part_gptr;  // tile p: groups [part_gptr[p], part_gptr[p+1])
+  std::vector<int> grp_row;    // group g: row positions [grp_row[g], grp_row[g+1])
+  std::vector<int> tperm;      // row position -> row
+  // per row position, maxe entries: >= 0 an LDS slot of the tile's own
+  // earlier rows ((group % kTileWindow) * lanes + row in group; the slot
+  // kTileWindow * lanes reads 0 and pads short rows), < 0 the halo slot
+  // -(code+1); tsrc: the entry's index in the row-order triangle (-1: pad)
+  std::vector<int> tcode, tsrc;
+  // group g's first reads from other tiles: [rf_ptr[g], rf_ptr[g+1]) of
+  // (rf_col: the row, rf_slot: its halo slot)
+  std::vector<int> rf_ptr, rf_col, rf_slot;
+};
+// ai/aj: 0-based strict triangle in row order; level: each row's level.
+// False (with the reason) when no tiling applies.
+bool build_tile_layout(int n, const int *ai, const int *aj, const int *level, int lanes,
+                       int max_parts, int max_rows, int max_halo, TileLayout &out,
+                       std::string &why);
 
 // seeded generators (gen.cpp)
 struct SplitMix64 {

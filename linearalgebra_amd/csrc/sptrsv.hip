@@ -195,6 +195,234 @@ __global__ __launch_bounds__(NT) void k_trsv_syncfree(
   }
 }
 
+// ------------------------------------------------------- tiled schedule
+// One workgroup per tile (see tiling.cpp): wavefront 0 solves the tile's
+// groups (at most 64 rows of one level each) in level order, one lane per
+// row, reading every x_j from LDS: its own results of the last kTileWindow
+// groups (`xl`) or the values of other tiles (`halo`). Wavefronts 1..NL
+// stage the groups ahead of it into a ring of LDS slots -- the rows' entry
+// codes and values, reciprocal diagonal, b and row index -- and poll the
+// other tiles' values each group reads first into `halo`; NL of them keep
+// NL groups' global loads in flight, so the staging keeps pace with the
+// solve. The dependency chain of a level is then a few LDS accesses (about
+// 0.1 us) instead of a hop through L2 between workgroups. Readiness between
+// the wavefronts is a per-slot group number (`s_ready`) and the count of
+// groups solved (`s_done`), both in LDS. A wavefront's LDS accesses are
+// carried out in the order it issues them, so the solving wavefront needs
+// no wait after its own writes (a compiler-only fence keeps the order in
+// the code); the staging wavefronts wait for their LDS writes before
+// publishing a slot. Every wait is bounded like the level-chunk kernel's;
+// a timeout sets *err.
+template <typename T> struct TileDt;
+template <> struct TileDt<double> { static constexpr la_dtype v = LA_F64; };
+template <> struct TileDt<float> { static constexpr la_dtype v = LA_F32; };
+template <> struct TileDt<bf16> { static constexpr la_dtype v = LA_BF16; };
+
+constexpr int kTileLanes = 64;
+constexpr int kTileRing = 24;     // ring slots (groups staged ahead)
+constexpr int kTileLoaders = 15;  // staging wavefronts per workgroup (16 in all)
+constexpr int kTileXl = kTileWindow * kTileLanes;  // LDS result slots; + 1 zero slot
+
+__device__ __forceinline__ void lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup", "local");
+}
+__device__ __forceinline__ void code_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+__device__ __forceinline__ int lds_load(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_store(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// spins until cond(*p) holds; false (with *err set) after kWaitTicks or
+// when another waiter has timed out. SLEEP: s_sleep units (64 clocks) per
+// poll -- the staging wavefronts poll slowly, so their reads do not queue
+// in the LDS ahead of the solving wavefront's
+template <int SLEEP, typename C>
+__device__ __forceinline__ bool spin_lds(const int *p, C cond, int *err) {
+  const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+  for (unsigned polls = 1;; ++polls) {
+    __builtin_amdgcn_s_sleep(SLEEP);
+    if (cond(lds_load(p))) return true;
+    if ((polls & 1023u) == 0 &&
+        (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+         __builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks)) {
+      __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+  }
+}
+
+template <int E, typename A>
+struct TileSlot {
+  // byte offsets inside one ring slot
+  static constexpr int code = 0;
+  static constexpr int val = code + E * kTileLanes * 4;
+  static constexpr int dv = val + E * kTileLanes * static_cast<int>(sizeof(A));
+  static constexpr int bv = dv + kTileLanes * static_cast<int>(sizeof(A));
+  static constexpr int pm = bv + kTileLanes * static_cast<int>(sizeof(A));
+  static constexpr int meta = pm + kTileLanes * 4;  // row count
+  static constexpr int bytes = (meta + 4 + 15) / 16 * 16;
+};
+
+inline size_t tile_lds_bytes(int h_max, size_t vsize, int slot_bytes) {
+  const size_t head = ((static_cast<size_t>(kTileXl) + 1 + h_max) * vsize + 15) / 16 * 16;
+  return head + static_cast<size_t>(kTileRing) * slot_bytes;
+}
+
+template <int E, int NL, typename T, typename V, typename A, bool DIAG>
+__global__ __launch_bounds__(kTileLanes * (1 + NL)) void k_trsv_tiled(
+    int nparts, const int *__restrict__ gptr, const int *__restrict__ grow,
+    const int *__restrict__ tperm, const int *__restrict__ tcode, const T *__restrict__ tval,
+    const T *__restrict__ tdinv, const int *__restrict__ rfptr, const int *__restrict__ rfcol,
+    const int *__restrict__ rfslot, int h_max, const V *__restrict__ bt, V *x, unsigned *ctl,
+    int *err) {
+  using S = TileSlot<E, A>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_ready[kTileRing];
+  __shared__ int s_done, s_part;
+  V *xl = reinterpret_cast<V *>(smem);
+  V *halo = xl + (kTileXl + 1);
+  unsigned char *ring = smem + ((static_cast<size_t>(kTileXl) + 1 + h_max) * sizeof(V) + 15) / 16 * 16;
+  const int wave = static_cast<int>(threadIdx.x) / kTileLanes;
+  const int lane = static_cast<int>(threadIdx.x) % kTileLanes;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      s_part = static_cast<int>(atomicAdd(&ctl[0], 1u));
+      s_done = 0;
+      xl[kTileXl] = V(0);  // the padding slot
+    }
+    if (threadIdx.x < kTileRing) s_ready[threadIdx.x] = -1;
+    __syncthreads();
+    const int p = s_part;
+    if (p >= nparts) break;
+    const int g0 = gptr[p], g1 = gptr[p + 1];
+    if (wave == 0) {
+      __builtin_amdgcn_s_setprio(3);  // issue priority over the staging wavefronts
+      // the slot of group i into registers: codes (padding for lanes past
+      // the group's rows, whose slot data is stale), values, b, 1/d, row
+      int c[E], nr = 0, pm = 0;
+      A a[E], bv = A(0), dv = A(1);
+      auto read_slot = [&](int i) {
+        const unsigned char *sb = ring + (i % kTileRing) * S::bytes;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          c[e] = reinterpret_cast<const int *>(sb + S::code)[e * kTileLanes + lane];
+          a[e] = reinterpret_cast<const A *>(sb + S::val)[e * kTileLanes + lane];
+        }
+        bv = reinterpret_cast<const A *>(sb + S::bv)[lane];
+        if constexpr (DIAG) dv = reinterpret_cast<const A *>(sb + S::dv)[lane];
+        pm = reinterpret_cast<const int *>(sb + S::pm)[lane];
+        nr = *reinterpret_cast<const int *>(sb + S::meta);
+      };
+      auto wait_slot = [&](int i) {
+        if (lds_load(&s_ready[i % kTileRing]) != i) {
+          spin_lds<0>(&s_ready[i % kTileRing], [i](int v) { return v == i; }, err);
+          lds_fence();
+        }
+        code_order();
+        read_slot(i);
+      };
+      const int ng = g1 - g0;
+      if (ng > 0) wait_slot(0);
+      for (int i = 0; i < ng; ++i) {
+        if (lane >= nr) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) c[e] = kTileXl;
+        }
+        V xv[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) xv[e] = c[e] >= 0 ? xl[c[e]] : halo[-c[e] - 1];
+        const A ai0 = a[0];
+        A acc = ai0 * A(xv[0]);
+#pragma unroll
+        for (int e = 1; e < E; ++e) acc += a[e] * A(xv[e]);
+        A v = bv - acc;
+        if constexpr (DIAG) v = v * dv;
+        const V out = V(v);
+        const bool live = lane < nr;
+        const int opm = pm;
+        // the next group's slot, read while this group's results are
+        // written (a slot read before its readiness is seen is read again)
+        int rdy = -1;
+        if (i + 1 < ng) {
+          rdy = lds_load(&s_ready[(i + 1) % kTileRing]);
+          code_order();
+          read_slot(i + 1);
+        }
+        if (live) {
+          xl[(i % kTileWindow) * kTileLanes + lane] = out;
+          publish(x + opm, out);
+        }
+        code_order();
+        if (lane == 0) lds_store(&s_done, i + 1);
+        if (i + 1 < ng && rdy != i + 1) wait_slot(i + 1);
+      }
+    } else {
+      for (int g = g0 + wave - 1; g < g1; g += NL) {
+        const int i = g - g0;
+        unsigned char *sb = ring + (i % kTileRing) * S::bytes;
+        // the slot is free once group i - kTileRing is solved
+        if (i >= kTileRing && lds_load(&s_done) < i - kTileRing + 1)
+          spin_lds<4>(&s_done, [i](int v) { return v >= i - kTileRing + 1; }, err);
+        lds_fence();
+        const int r0 = grow[g], nr = grow[g + 1] - r0;
+        const int f0 = rfptr[g], f1 = rfptr[g + 1];
+        // the other tiles' values this group reads first (one per lane in
+        // the common case), loaded alongside the rows' data
+        V hv = V(0);
+        if (f0 + lane < f1) hv = load_ready(x + rfcol[f0 + lane]);
+        if (lane < nr) {
+          const int k = r0 + lane;
+          int *code = reinterpret_cast<int *>(sb + S::code);
+          A *val = reinterpret_cast<A *>(sb + S::val);
+          int cc[E];
+          A vv[E];
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            cc[e] = tcode[static_cast<int64_t>(k) * E + e];
+            vv[e] = A(ldt(tval, static_cast<int64_t>(k) * E + e));
+          }
+          A d = A(1);
+          if constexpr (DIAG) d = A(ldt(tdinv, k));
+          const int r = tperm[k];
+          const A bb = A(bt[k]);
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            code[e * kTileLanes + lane] = cc[e];
+            val[e * kTileLanes + lane] = vv[e];
+          }
+          reinterpret_cast<A *>(sb + S::dv)[lane] = d;
+          reinterpret_cast<int *>(sb + S::pm)[lane] = r;
+          reinterpret_cast<A *>(sb + S::bv)[lane] = bb;
+        }
+        if (lane == 0) *reinterpret_cast<int *>(sb + S::meta) = nr;
+        for (int t = f0 + lane; t < f1; t += kTileLanes) {
+          V xv = t == f0 + lane ? hv : load_ready(x + rfcol[t]);
+          if (is_sentinel(xv)) {
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            for (unsigned polls = 1; is_sentinel(xv); ++polls) {
+              __builtin_amdgcn_s_sleep(1);
+              xv = load_ready(x + rfcol[t]);
+              if ((polls & 255u) == 0 &&
+                  (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                   __builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks)) {
+                __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                xv = V(0);
+                break;
+              }
+            }
+          }
+          halo[rfslot[t]] = xv;
+        }
+        lds_fence();
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) lds_store(&s_ready[i % kTileRing], i);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 inline int pick_G(double avg) {
   int G = 1;
   while (G < 64 && G < avg) G <<= 1;
@@ -239,9 +467,55 @@ void sf_launch(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x, hipSt
                      P.nchunks, P.chunk_start.as<int>(), rp, cl, val, diag, pm, b, x, ctl, err);
 }
 
+// x <- sentinel, bt <- b in tiled order, control words reset (the tiled
+// schedule's stand-in for k_fill_sentinel)
+template <typename V>
+__global__ void k_tile_prep(int64_t n, const int *__restrict__ tperm, const V *__restrict__ b,
+                            V *bt, V *x, unsigned *ctl) {
+  using U = typename Sentinel<V>::U;
+  U *xb = reinterpret_cast<U *>(x);
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * kB) {
+    bt[t] = b[tperm[t]];
+    xb[t] = Sentinel<V>::bits;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 16) ctl[threadIdx.x] = 0;
+}
+
+template <int E, typename T, typename V, typename A, bool DIAG>
+void tiled_launch(TrsvPlan &P, const V *b, V *x, hipStream_t s) {
+  auto kern = k_trsv_tiled<E, kTileLoaders, T, V, A, DIAG>;
+  const size_t lds = tile_lds_bytes(P.t_hmax, sizeof(V), TileSlot<E, A>::bytes);
+  static size_t lds_set = 0;  // per instantiation: the largest size allowed so far
+  if (lds > lds_set) {
+    LA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(lds)));
+    lds_set = lds;
+  }
+  const T *tv = P.nnz ? static_cast<const T *>(P.t_val.get(TileDt<T>::v)) : nullptr;
+  const T *td = DIAG ? static_cast<const T *>(P.t_dinv.get(TileDt<T>::v)) : nullptr;
+  V *bt = P.t_bt.as<V>();
+  const unsigned fg = static_cast<unsigned>(std::min<int64_t>(4096, (P.n + kB - 1) / kB));
+  hipLaunchKernelGGL(k_tile_prep<V>, dim3(std::max(fg, 1u)), dim3(kB), 0, s,
+                     static_cast<int64_t>(P.n), P.t_perm.as<int>(), b, bt, x,
+                     P.counter.as<unsigned>());
+  hipLaunchKernelGGL(kern, dim3(P.t_wgs), dim3(kTileLanes * (1 + kTileLoaders)), lds, s,
+                     P.t_nparts, P.t_gptr.as<int>(), P.t_grow.as<int>(), P.t_perm.as<int>(),
+                     P.t_code.as<int>(), tv, td, P.t_rfptr.as<int>(), P.t_rfcol.as<int>(),
+                     P.t_rfslot.as<int>(), P.t_hmax, bt, x, P.counter.as<unsigned>(),
+                     P.err_flag.as<int>());
+}
+
 template <typename T, typename V, typename A, bool DIAG>
 void launch_syncfree(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x, hipStream_t s) {
   const int64_t n = P.n;
+  if (P.tiled) {
+    if (P.t_maxe == 4) tiled_launch<4, T, V, A, DIAG>(P, b, x, s);
+    else tiled_launch<kTileMaxEntries, T, V, A, DIAG>(P, b, x, s);
+    LA_HIP(hipGetLastError());
+    return;
+  }
   unsigned fg = static_cast<unsigned>(std::min<int64_t>(4096, (n + kB - 1) / kB));
   if (fg == 0) fg = 1;
   hipLaunchKernelGGL(k_fill_sentinel<V>, dim3(fg), dim3(kB), 0, s, n, x, P.counter.as<unsigned>());
@@ -434,6 +708,109 @@ void jacobi_solve(TrsvPlan &T, la_dtype val, la_dtype vec, const void *b, void *
   jac_scale(n, val, vec, d, tmp, x, s);
 }
 
+// tiled order <- level order: the plan's values (entry source index, -1 for
+// padding) and reciprocal diagonal
+__global__ void k_tile_gather(int64_t m, const int *__restrict__ src, const double *__restrict__ f,
+                              double *out) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < m) out[t] = src[t] >= 0 ? f[src[t]] : 0.0;
+}
+
+void tiled_refresh(TrsvPlan &T, hipStream_t s) {
+  if (!T.tiled) return;
+  const int64_t m = static_cast<int64_t>(T.n) * T.t_maxe;
+  if (T.nnz) {
+    double *tv = T.t_val.v[LA_F64].as<double>();
+    hipLaunchKernelGGL(k_tile_gather, dim3(static_cast<unsigned>((m + 255) / 256)), dim3(256), 0,
+                       s, m, T.t_lsrc.as<int>(), T.val.v[LA_F64].as<double>(), tv);
+    LA_HIP(hipGetLastError());
+    T.t_val.set_from_device_f64(tv, m, s);
+  }
+  if (!T.unit) {
+    double *td = T.t_dinv.v[LA_F64].as<double>();
+    hipLaunchKernelGGL(k_tile_gather, dim3(static_cast<unsigned>((T.n + 255) / 256)), dim3(256),
+                       0, s, static_cast<int64_t>(T.n), T.t_dpos.as<int>(),
+                       T.diag.v[LA_F64].as<double>(), td);
+    LA_HIP(hipGetLastError());
+    T.t_dinv.set_from_device_f64(td, T.n, s);
+  }
+}
+
+// The tiled schedule when the triangle's pattern is recognised as a grid
+// and a tile's rows, faces and staging ring fit one workgroup's LDS; one
+// workgroup per tile, at most one per CU, so every tile is resident.
+void build_tiled(TrsvPlan &P, const std::vector<int> &iperm, const std::vector<int> &lrp,
+                 hipStream_t s) {
+  const int n = P.n;
+  std::vector<int> level(static_cast<size_t>(n)), lpos(static_cast<size_t>(n));
+  for (int l = 0; l < P.nlevels; ++l)
+    for (int k = P.level_ptr[l]; k < P.level_ptr[l + 1]; ++k) {
+      level[iperm[k]] = l;
+      lpos[iperm[k]] = k;
+    }
+  int maxe = 0;
+  for (int i = 0; i < n; ++i) maxe = std::max(maxe, P.h_ai[i + 1] - P.h_ai[i]);
+  const int slot = maxe <= 4 ? TileSlot<4, double>::bytes : TileSlot<kTileMaxEntries, double>::bytes;
+  int dev = 0, ncu = 0, budget = 0;
+  LA_HIP(hipGetDevice(&dev));
+  LA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  LA_HIP(hipDeviceGetAttribute(&budget, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const int64_t fixed = static_cast<int64_t>(tile_lds_bytes(0, sizeof(double), slot));
+  const int64_t avail = static_cast<int64_t>(budget) - fixed - 64;
+  if (avail < 1024 || ncu < 1) {
+    P.t_why = "LDS too small for the tiled schedule";
+    return;
+  }
+  const int max_halo = static_cast<int>(avail / 8);
+  TileLayout L;
+  if (!build_tile_layout(n, P.h_ai.data(), P.h_aj.data(), level.data(), kTileLanes, ncu, n,
+                         max_halo, L, P.t_why))
+    return;
+  if (tile_lds_bytes(L.h_max, sizeof(double), slot) > static_cast<size_t>(budget)) {
+    P.t_why = "tile does not fit LDS";
+    return;
+  }
+  const int E = L.maxe;
+  const int64_t m = static_cast<int64_t>(n) * E;
+  std::vector<int> lsrc(static_cast<size_t>(m)), dpos(static_cast<size_t>(n));
+  std::vector<double> tv(static_cast<size_t>(m)), td(P.unit ? 0 : static_cast<size_t>(n));
+  for (int k = 0; k < n; ++k) {
+    const int r = L.tperm[k];
+    dpos[k] = lpos[r];
+    if (!P.unit) td[k] = 1.0 / P.h_diag[r];
+    for (int e = 0; e < E; ++e) {
+      const int q = L.tsrc[static_cast<size_t>(k) * E + e];
+      lsrc[static_cast<size_t>(k) * E + e] = q >= 0 ? lrp[lpos[r]] + (q - P.h_ai[r]) : -1;
+      tv[static_cast<size_t>(k) * E + e] = q >= 0 ? P.h_av[q] : 0.0;
+    }
+  }
+  auto up = [&](DevBuf &d, const std::vector<int> &h) {
+    d.alloc(sizeof(int) * std::max<size_t>(h.size(), 1));
+    d.upload(h.data(), sizeof(int) * h.size(), s);
+  };
+  up(P.t_gptr, L.part_gptr);
+  up(P.t_grow, L.grp_row);
+  up(P.t_perm, L.tperm);
+  up(P.t_code, L.tcode);
+  up(P.t_rfptr, L.rf_ptr);
+  up(P.t_rfcol, L.rf_col);
+  up(P.t_rfslot, L.rf_slot);
+  up(P.t_lsrc, lsrc);
+  up(P.t_dpos, dpos);
+  P.t_bt.alloc(sizeof(double) * static_cast<size_t>(n));
+  if (P.nnz) P.t_val.set_from_host(tv.data(), m, s);
+  if (!P.unit) P.t_dinv.set_from_host(td.data(), n, s);
+  P.t_nparts = L.nparts;
+  P.t_ngroups = L.ngroups;
+  P.t_mmax = L.m_max;
+  P.t_hmax = L.h_max;
+  P.t_maxe = E;
+  P.t_tau = L.tau;
+  P.t_remote = L.remote_frac;
+  P.t_wgs = static_cast<unsigned>(std::min(L.nparts, ncu));
+  P.tiled = true;
+}
+
 hipStream_t capture_stream() {
   static thread_local hipStream_t cs = nullptr;
   if (!cs) LA_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
@@ -540,6 +917,8 @@ std::unique_ptr<TrsvPlan> make_trsv(int n, int base, const int *ai, const int *a
   LA_HIP(hipMemsetAsync(P->counter.p, 0, sizeof(unsigned) * 16, s));
   P->val.set_from_host(vl.data(), nnz, s);
   if (diag) P->diag.set_from_host(dg.data(), n, s);
+  const char *te = std::getenv("LAAMD_SF_TILED");
+  if (!(te && te[0] == '0') && n > 0) build_tiled(*P, iperm, rp, s);
   LA_HIP(hipStreamSynchronize(s));
   return P;
 }
@@ -548,6 +927,10 @@ void trsv_add_dtype(TrsvPlan &T, la_dtype dt) {
   if (T.nnz) T.val.add(dt, T.stream);
   else if (!T.val.has[LA_F64]) { /* empty triangle: nothing stored */ }
   if (!T.unit) T.diag.add(dt, T.stream);
+  if (T.tiled) {
+    if (T.nnz) T.t_val.add(dt, T.stream);
+    if (!T.unit) T.t_dinv.add(dt, T.stream);
+  }
   if (T.transposed) trsv_add_dtype(*T.transposed, dt);
   LA_HIP(hipStreamSynchronize(T.stream));
 }
@@ -676,6 +1059,7 @@ void trsv_refresh(TrsvPlan &T, const double *f, hipStream_t s) {
     LA_HIP(hipGetLastError());
     T.diag.set_from_device_f64(d, T.n, s);
   }
+  tiled_refresh(T, s);
   if (T.jac) jacobi_refresh(T, f, s);
   if (T.transposed) trsv_refresh(*T.transposed, f, s);
 }

@@ -216,3 +216,90 @@ def test_rejects_non_strict_triangle():
     Lw = orc.split_triangle(A, False)  # keeps the diagonal
     with pytest.raises(la.LaError):
         la.TriangularSolve(as_lib_csr(Lw), None, upper=False)
+
+
+# ------------------------------------------------------------- tiled schedule
+
+def _grid_cases():
+    """Triangles whose pattern is a grid (the tiled sync-free schedule's
+    case): natural and RCM order, 2-D and 3-D, IC(0) and ILU(0)."""
+    def rcm(A):
+        ip, p = orc.rcm(A)
+        return orc.permute(A, ip, p)
+    return [("poisson2d_96_natural", lambda: _o(la.gen.poisson2d(96))),
+            ("poisson2d_80_rcm", lambda: rcm(_o(la.gen.poisson2d(80)))),
+            ("laplace3d_24_rcm", lambda: rcm(_o(la.gen.laplace3d(24)))),
+            ("laplace3d_20x16x12_natural", lambda: _o(la.gen.laplace3d(20, 16, 12)))]
+
+
+@pytest.mark.parametrize("name,make", _grid_cases(), ids=[c[0] for c in _grid_cases()])
+def test_tiled_schedule_ic0_apply(dev, name, make):
+    """The sync-free mode runs the tiled schedule on grid patterns (both
+    triangles, any row order) and gives the oracle's apply to 1e-12 in FP64,
+    the level-launch result to 1e-13, and the FP32 apply to the FP32
+    bound; also transposed (A symmetric)."""
+    import torch
+    A = make()
+    U = orc.split_triangle(A, True)
+    R, ok = orc.icc(U, 0)
+    P = la.Preconditioner(as_lib_csr(R), "ic")
+    tl, tu = P.tiles()
+    print("case", name, "n", A.n, "tiles", tl, tu, flush=True)
+    assert tl > 1 and tu > 1
+    b = la.gen.uniform(A.n, 12)
+    ref = orc.prec_apply(A, "ic", R, b)
+    bt = torch.tensor(b, device=dev)
+    z = torch.zeros_like(bt)
+    P.apply(bt, z, mode=la.TRSV_SYNCFREE)
+    assert _close(z.cpu().numpy(), ref, 1e-12)
+    zl = torch.zeros_like(bt)
+    P.apply(bt, zl, mode=la.TRSV_LEVEL)
+    assert _close(z.cpu().numpy(), zl.cpu().numpy(), 1e-13)
+    for _ in range(3):  # repeated solves reuse the plan's control words
+        z2 = torch.zeros_like(bt)
+        P.apply(bt, z2, mode=la.TRSV_SYNCFREE)
+        assert torch.equal(z2, z)
+    z32 = torch.zeros(A.n, dtype=torch.float32, device=dev)
+    P.apply(bt.float(), z32, mode=la.TRSV_SYNCFREE, val=la.F32)
+    assert np.linalg.norm(z32.double().cpu().numpy() - ref) <= 1e-4 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("name,make", _grid_cases()[:3], ids=[c[0] for c in _grid_cases()[:3]])
+def test_tiled_schedule_ilu0_unit_lower(dev, name, make):
+    """ILU(0) on a grid: the unit lower triangle and the upper one with its
+    diagonal both tile; forward and backward solves match the oracle."""
+    import torch
+    F = orc.iluk(make(), 0)[0]
+    L, D, U = orc.split_ldu(F)
+    fw = la.TriangularSolve(as_lib_csr(L), None, upper=False)
+    bw = la.TriangularSolve(as_lib_csr(U), D, upper=True)
+    assert fw.info()["tiled"] and bw.info()["tiled"]
+    b = la.gen.uniform(F.n, 13)
+    bt = torch.tensor(b, device=dev)
+    x = torch.zeros_like(bt)
+    fw.solve(bt, x, mode=la.TRSV_SYNCFREE)
+    assert _close(x.cpu().numpy(), orc.forward_sub(L, b), 1e-12)
+    bw.solve(bt, x, mode=la.TRSV_SYNCFREE)
+    assert _close(x.cpu().numpy(), orc.backward_sub(U, D, b), 1e-12)
+
+
+def test_tiled_schedule_off_and_non_grid(dev, monkeypatch):
+    """LAAMD_SF_TILED=0 at plan creation keeps the level-chunk schedule;
+    a pattern that is not a grid keeps it too; both solve exactly."""
+    import torch
+    A = _o(la.gen.laplace3d(16))
+    R, ok = orc.icc(orc.split_triangle(A, True), 0)
+    monkeypatch.setenv("LAAMD_SF_TILED", "0")
+    P0 = la.Preconditioner(as_lib_csr(R), "ic")
+    monkeypatch.delenv("LAAMD_SF_TILED")
+    P1 = la.Preconditioner(as_lib_csr(R), "ic")
+    assert P0.tiles() == (0, 0) and min(P1.tiles()) > 0
+    b = torch.tensor(la.gen.uniform(A.n, 14), device=dev)
+    z0, z1 = torch.zeros_like(b), torch.zeros_like(b)
+    P0.apply(b, z0, mode=la.TRSV_SYNCFREE)
+    P1.apply(b, z1, mode=la.TRSV_SYNCFREE)
+    assert _close(z1.cpu().numpy(), z0.cpu().numpy(), 1e-13)
+    F = orc.iluk(_o(la.gen.powerlaw(3000, 8.0, 1.0, 400, seed=2)), 0)[0]
+    L, D, U = orc.split_ldu(F)
+    fw = la.TriangularSolve(as_lib_csr(L), None, upper=False)
+    assert not fw.info()["tiled"]
