@@ -312,7 +312,6 @@ int la_mat_set_values(la_mat_t A, const double *av) {
   LA_TRY
   LA_CHECK_ARG(A && (av || A->m->nnz == 0), "bad argument");
   A->m->vals.set_from_host(av, A->m->nnz, A->m->stream);
-  A->m->transposed.reset();
   LA_HIP(hipStreamSynchronize(A->m->stream));
   LA_CATCH
 }
@@ -320,7 +319,6 @@ int la_mat_set_values_device(la_mat_t A, const double *av) {
   LA_TRY
   LA_CHECK_ARG(A && (av || A->m->nnz == 0), "bad argument");
   A->m->vals.set_from_device_f64(av, A->m->nnz, A->m->stream);
-  A->m->transposed.reset();
   LA_CATCH
 }
 int la_mat_set_stream(la_mat_t A, void *s) {

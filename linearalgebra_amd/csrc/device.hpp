@@ -32,8 +32,13 @@ struct Mat {
   bool sell_col16 = false; // columns stored as 16-bit offsets from the row
   DevBuf sell_ptr, sell_w, sell_col, sell_src, sell_val[3];
   uint64_t sell_ver[3] = {0, 0, 0};
-  // A^T, built on first transposed product
+  // A^T, its pattern built on first transposed product; its values are
+  // gathered on the device from A's (t_src: the entry of A each entry of
+  // A^T holds) whenever A's values changed (t_ver: the version gathered)
   std::unique_ptr<Mat> transposed;
+  DevBuf t_src;
+  uint64_t t_ver = ~0ull;
+  int sym_pattern = -1;  // structurally symmetric: -1 not known yet
   // host copy of the pattern, kept for the transpose and the value map
   std::vector<int> h_ai, h_aj;
 };
@@ -58,9 +63,12 @@ void spmv(Mat &A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, dou
           const void *x, double beta, const void *z, void *y, hipStream_t s);
 void spmv_transpose(Mat &A, la_dtype val, la_dtype vec, double alpha, const void *x,
                     double beta, void *y, hipStream_t s);
-// A^T as its own matrix (host stable transpose of the FP64 values, built once;
-// lower-precision copies follow A's)
+// A^T as its own matrix: the pattern once (stable host transpose), the
+// values gathered on the device whenever A's change; lower-precision copies
+// follow A's
 Mat &ensure_transposed(Mat &A, hipStream_t s);
+// A's pattern is structurally symmetric (checked once)
+bool mat_symmetric_pattern(Mat &A);
 void csr_value_grad(Mat &A, const double *lambda, const double *x, double *g, hipStream_t s);
 
 // -------------------------------------------------------------- SpTRSV plan

@@ -43,6 +43,5 @@ void mat_dtad(Mat &A, const double *d, hipStream_t s) {
                      A.ai.as<int>(), A.aj.as<int>(), v, d);
   LA_HIP(hipGetLastError());
   A.vals.set_from_device_f64(v, A.nnz, s);
-  A.transposed.reset();
 }
 }  // namespace laamd

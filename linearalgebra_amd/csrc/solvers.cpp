@@ -512,9 +512,13 @@ int fgmres_solve_right(Mat &A, Prec *P, Prec *R, const la_solver_opts &o, const 
 namespace laamd {
 Mat &transposed_of(Mat &A) { return ensure_transposed(A, A.stream); }
 
+// CG presumes a symmetric A, for which A^T = A: on a structurally symmetric
+// pattern the transposed solve runs on A itself (no transposed copy); the
+// preconditioner's transposed apply is still used (an IC apply is its own
+// transpose).
 int pcg_solve_transpose(Mat &A, Prec *P, const la_solver_opts &o, const double *b, double *x,
                         la_solver_result *res) {
-  Mat &At = transposed_of(A);
+  Mat &At = mat_symmetric_pattern(A) ? A : transposed_of(A);
   TransposeScope scope(true);
   return pcg_solve(At, P, o, b, x, res);
 }

@@ -663,25 +663,55 @@ void spmv(Mat &A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, dou
   }
 }
 
+namespace {
+// values of A^T from A's: out[t] = v[src[t]]
+__global__ void k_gather_values(int64_t m, const int *__restrict__ src,
+                                const double *__restrict__ v, double *out) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < m) out[t] = v[src[t]];
+}
+}  // namespace
+
 Mat &ensure_transposed(Mat &A, hipStream_t s) {
   if (!A.transposed) {
-    // download the FP64 values, transpose on the host (stable), upload
-    std::vector<double> hv(static_cast<size_t>(A.nnz));
-    if (A.nnz)
-      LA_HIP(hipMemcpyAsync(hv.data(), A.vals.get(LA_F64), sizeof(double) * A.nnz,
-                            hipMemcpyDeviceToHost, s));
-    LA_HIP(hipStreamSynchronize(s));
+    // the pattern of A^T and, for each of its entries, the entry of A it
+    // holds (a stable transpose of the entry indices), once per pattern;
+    // the values follow on the device
     std::vector<int> ti(static_cast<size_t>(A.cols) + 1), tj(static_cast<size_t>(A.nnz));
-    std::vector<double> tv(static_cast<size_t>(A.nnz));
-    transpose(A.rows, A.cols, 0, A.h_ai.data(), A.h_aj.data(), hv.data(), ti.data(), tj.data(),
-              tv.data());
-    A.transposed = make_mat(A.cols, A.rows, 0, ti.data(), tj.data(), tv.data(), s);
+    std::vector<double> idx(static_cast<size_t>(A.nnz)), tidx(static_cast<size_t>(A.nnz));
+    for (int64_t q = 0; q < A.nnz; ++q) idx[q] = static_cast<double>(q);  // exact below 2^53
+    transpose(A.rows, A.cols, 0, A.h_ai.data(), A.h_aj.data(), idx.data(), ti.data(), tj.data(),
+              tidx.data());
+    std::vector<int> src(tidx.begin(), tidx.end());
+    std::vector<double> zeros(static_cast<size_t>(A.nnz), 0.0);
+    A.transposed = make_mat(A.cols, A.rows, 0, ti.data(), tj.data(), zeros.data(), s);
+    A.t_src.alloc(sizeof(int) * std::max<size_t>(src.size(), 1));
+    A.t_src.upload(src.data(), sizeof(int) * src.size(), s);
+    A.t_ver = ~0ull;
   }
-  A.transposed->stream = A.stream;
+  Mat &T = *A.transposed;
+  T.stream = A.stream;
+  if (A.t_ver != A.vals.version) {
+    if (A.nnz) {
+      double *tv = T.vals.v[LA_F64].as<double>();
+      hipLaunchKernelGGL(k_gather_values, dim3(static_cast<unsigned>((A.nnz + 255) / 256)),
+                         dim3(256), 0, s, A.nnz, A.t_src.as<int>(),
+                         static_cast<const double *>(A.vals.get(LA_F64)), tv);
+      LA_HIP(hipGetLastError());
+      T.vals.set_from_device_f64(tv, A.nnz, s);
+    }
+    A.t_ver = A.vals.version;
+  }
   for (int dt = 1; dt < 3; ++dt)
-    if (A.vals.has[dt] && !A.transposed->vals.has[dt])
-      A.transposed->vals.add(static_cast<la_dtype>(dt), s);
-  return *A.transposed;
+    if (A.vals.has[dt] && !T.vals.has[dt]) T.vals.add(static_cast<la_dtype>(dt), s);
+  return T;
+}
+
+bool mat_symmetric_pattern(Mat &A) {
+  if (A.sym_pattern < 0)
+    A.sym_pattern = A.rows == A.cols && pattern_symmetric(A.rows, 0, A.h_ai.data(), A.h_aj.data())
+                        ? 1 : 0;
+  return A.sym_pattern == 1;
 }
 
 void spmv_transpose(Mat &A, la_dtype val, la_dtype vec, double alpha, const void *x,

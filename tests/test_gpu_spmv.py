@@ -223,6 +223,31 @@ def test_spmv_transpose(dev, name, make):
     assert np.all(np.abs(yt.cpu().numpy() - ref) <= 1e-12 * scale + 1e-300)
 
 
+def test_spmv_transpose_follows_value_changes(dev):
+    """A^T's values are gathered on the device from A's whenever they change
+    (host or device update, FP32 copy included); the pattern is kept."""
+    import torch
+    A = read_mtx("rdist1.mtx")
+    x = la.gen.uniform(A.n, 3)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    xt = torch.tensor(x, device=dev)
+    rng = np.random.default_rng(4)
+    for step, on_device in enumerate([False, True, False]):
+        av = rng.standard_normal(A.nnz)
+        if on_device:
+            dA.set_values(torch.tensor(av, device=dev))
+        else:
+            dA.set_values(av)
+        At = orc.transpose(orc.CSR(A.n, A.ai, A.aj, av))
+        y = torch.zeros(A.n, dtype=torch.float64, device=dev)
+        dA.spmv_transpose(xt, y)
+        ref = orc.spmv(At, x)
+        assert np.all(np.abs(y.cpu().numpy() - ref) <= 1e-12 * _absbound(At, x) + 1e-300), step
+        y32 = torch.zeros(A.n, dtype=torch.float32, device=dev)
+        dA.spmv_transpose(xt.float(), y32, val=la.F32)
+        assert np.all(np.abs(y32.double().cpu().numpy() - ref) <= 2e-6 * _absbound(At, x) + 1e-30)
+
+
 def test_value_grad(dev):
     import torch
     A = read_mtx("rdist1.mtx")
