@@ -56,7 +56,7 @@ METRIC = ("SpMV GB/s (%HBM peak); PCG time to 1e-10: mixed FP32/BF16 vs FP64, "
           "1 GPU")
 HBM_PEAK_GBS = 8000.0
 # SPMV_* codes (ROW 0, SEGSUM 1, ALBUS 2, AUTO 3, SELL 4)
-ALG_NAMES = {0: "row", 1: "segsum", 2: "albus", 3: "auto", 4: "sell"}
+ALG_NAMES = {0: "row", 1: "segsum", 2: "albus", 3: "auto", 4: "sell", 5: "colblock"}
 ALG_DESC = {0: "one wavefront per row", 2: "nnz-balanced segment kernel, 512 nnz per "
             "wavefront, + boundary-row fix-up",
             4: "sliced ELL, 64-row column-major slices, one lane per row, 16-bit column "
@@ -381,19 +381,22 @@ def bench_c3(la, torch, dev, stream, args):
     t0 = time.time()
     A = la.gen.powerlaw(4_000_000, 16.0, 1.0, 20000, seed=3)
     prep = round(time.time() - t0, 3)
-    if args.pmc_probe:  # 4 FP64 SpMVs + a 1 GiB copy, for rocprofv3 --pmc
+    if args.pmc_probe:  # 4 FP64 SpMVs of each form + a 1 GiB copy, for rocprofv3 --pmc
         d = la.DeviceMatrix(A, stream=stream)
         x = torch.tensor(la.gen.uniform(A.n, 7), device=dev)
         y = torch.zeros_like(x)
-        for _ in range(4):
-            d.spmv(x, y, alg=la.SPMV_ALBUS)
+        for alg in (la.SPMV_ALBUS, la.SPMV_AUTO):
+            for _ in range(4):
+                d.spmv(x, y, alg=alg)
         big = torch.empty(1 << 27, dtype=torch.float64, device=dev)
         torch.empty_like(big).copy_(big)
         torch.cuda.synchronize()
-        return {"probe": "c3 spmv x4 + copy 1GiB", "nnz": A.nnz}
+        return {"probe": "c3 spmv albus x4, auto (column blocks) x4, copy 1GiB", "nnz": A.nnz}
     rl = np.diff(A.ai)
     mats = [la.DeviceMatrix(A, stream=stream) for _ in range(2)]
     algs = [("auto", la.SPMV_AUTO), ("albus", la.SPMV_ALBUS), ("row", la.SPMV_ROW)]
+    # the column-blocked form AUTO picks here: blocks of the matrix's columns
+    # (4 MB of FP64 x each), one ALBUS pass per block adding into y
     tab = _spmv_table(la, torch, A, mats, dev, stream, algs,
                       [(la.F64, "f64"), (la.F32, "f32"), (la.BF16, "bf16")], 20)
     return {"config": "c3 power-law n=4M", "n": A.n, "nnz": A.nnz,

@@ -52,8 +52,13 @@ typedef enum {
   LA_SPMV_SEGSUM = 1,
   LA_SPMV_ALBUS = 2,
   LA_SPMV_AUTO = 3,
-  LA_SPMV_SELL = 4 /* sliced ELL copy, one lane per row (no counterpart in the
-                      reference; AUTO picks it for near-uniform row lengths) */
+  LA_SPMV_SELL = 4, /* sliced ELL copy, one lane per row (no counterpart in the
+                       reference; AUTO picks it for near-uniform row lengths) */
+  LA_SPMV_COLBLOCK = 5 /* ALBUS on column blocks of the matrix, one block (a
+                          few MB of x, held by the L2 caches) after the other,
+                          each adding into y (no counterpart in the reference;
+                          AUTO picks it for ragged rows whose columns are
+                          scattered over an x larger than the L2 caches) */
 } la_spmv_alg;
 
 /* triangle_solve.hpp:147-208 OptimizedTriangularSolve modes: Barrier ->
@@ -194,8 +199,10 @@ int la_mat_info(la_mat_t A, int *rows, int *cols, int64_t *nnz);
  * nnz-balanced segment table. Called implicitly on first use. */
 int la_spmv_preprocess(la_mat_t A, la_spmv_alg alg);
 /* The kernel family a request runs on for this matrix: AUTO and SELL give
- * SELL when the sliced-ELL padding stays small, else ALBUS; SEGSUM runs on
- * ALBUS's kernels; an empty matrix runs ROW. Builds what the choice needs.
+ * SELL when the sliced-ELL padding stays small; otherwise AUTO gives
+ * COLBLOCK for scattered columns over a large x, else ALBUS; SELL falls back
+ * to ALBUS; SEGSUM runs on ALBUS's kernels; an empty matrix runs ROW.
+ * Builds what the choice needs.
  * No counterpart in the reference (its algorithm is a template argument). */
 int la_spmv_resolve(la_mat_t A, la_spmv_alg alg, int *chosen);
 /* The sliced-ELL copy (built on demand): *built 1 if the matrix has one,

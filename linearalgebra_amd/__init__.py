@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (BF16, F32, F64, LOWER, PREC_FP64, PREC_MIXED_BF16,
-                   PREC_MIXED_BF16_F32P, PREC_MIXED_F32, PREC_MIXED_F32_BF16P, SPMV_ALBUS, SPMV_AUTO, SPMV_ROW, SPMV_SELL,
+                   PREC_MIXED_BF16_F32P, PREC_MIXED_F32, PREC_MIXED_F32_BF16P, SPMV_ALBUS, SPMV_AUTO, SPMV_COLBLOCK, SPMV_ROW, SPMV_SELL,
                    SPMV_SEGSUM, STOP_RECURRENCE, STOP_TRUE_RESIDUAL,
                    TRSV_JACOBI, TRSV_LEVEL, TRSV_SYNCFREE, UPPER, LaError, SolverOpts,
                    SolverResult, check)

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "liblaamd.so")
 LA_OK, LA_ERR_ARG, LA_ERR_HIP, LA_ERR_ALLOC = 0, 1, 2, 3
 LA_ERR_NOT_CONVERGED, LA_ERR_NUMERIC, LA_ERR_STATE, LA_ERR_TIMEOUT = 4, 5, 6, 7
 F64, F32, BF16 = 0, 1, 2
-SPMV_ROW, SPMV_SEGSUM, SPMV_ALBUS, SPMV_AUTO, SPMV_SELL = 0, 1, 2, 3, 4
+SPMV_ROW, SPMV_SEGSUM, SPMV_ALBUS, SPMV_AUTO, SPMV_SELL, SPMV_COLBLOCK = 0, 1, 2, 3, 4, 5
 TRSV_LEVEL, TRSV_SYNCFREE, TRSV_JACOBI = 0, 1, 2
 LOWER, UPPER = 0, 1
 PREC_FP64, PREC_MIXED_F32, PREC_MIXED_BF16, PREC_MIXED_BF16_F32P, PREC_MIXED_F32_BF16P = 0, 1, 2, 3, 4

@@ -339,16 +339,17 @@ int la_mat_info(la_mat_t A, int *rows, int *cols, int64_t *nnz) {
 int la_spmv_preprocess(la_mat_t A, la_spmv_alg alg) {
   LA_TRY
   LA_CHECK_ARG(A, "null handle");
-  LA_CHECK_ARG(alg >= LA_SPMV_ROW && alg <= LA_SPMV_SELL, "bad SpMV algorithm");
+  LA_CHECK_ARG(alg >= LA_SPMV_ROW && alg <= LA_SPMV_COLBLOCK, "bad SpMV algorithm");
   if (alg == LA_SPMV_SEGSUM || alg == LA_SPMV_ALBUS || alg == LA_SPMV_AUTO)
     spmv_prepare_segments(*A->m);
-  if (alg == LA_SPMV_SELL || alg == LA_SPMV_AUTO) spmv_prepare_sell(*A->m);
+  if (alg == LA_SPMV_SELL || alg == LA_SPMV_AUTO || alg == LA_SPMV_COLBLOCK)
+    (void)spmv_resolve(*A->m, alg);
   LA_CATCH
 }
 int la_spmv_resolve(la_mat_t A, la_spmv_alg alg, int *chosen) {
   LA_TRY
   LA_CHECK_ARG(A && chosen, "bad argument");
-  LA_CHECK_ARG(alg >= LA_SPMV_ROW && alg <= LA_SPMV_SELL, "bad SpMV algorithm");
+  LA_CHECK_ARG(alg >= LA_SPMV_ROW && alg <= LA_SPMV_COLBLOCK, "bad SpMV algorithm");
   *chosen = static_cast<int>(spmv_resolve(*A->m, alg));
   LA_CATCH
 }

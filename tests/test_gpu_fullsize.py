@@ -115,8 +115,9 @@ def test_c2_ic0_apply_and_pcg_full(dev):
 def test_c3_powerlaw_spmv_full(dev):
     A = la.gen.powerlaw(4_000_000, 16.0, 1.0, 20000, seed=3)
     assert A.n == 4_000_000 and 60e6 < A.nnz < 70e6
-    _check_spmv(dev, A, [la.SPMV_ALBUS, la.SPMV_ROW],
-                [(la.F64, 1e-12), (la.F32, 2e-6), (la.BF16, BF16_BOUND)])
+    d = _check_spmv(dev, A, [la.SPMV_AUTO, la.SPMV_ALBUS, la.SPMV_ROW],
+                    [(la.F64, 1e-12), (la.F32, 2e-6), (la.BF16, BF16_BOUND)])
+    assert d.resolve(la.SPMV_AUTO) == la.SPMV_COLBLOCK
 
 
 def test_c4_spmv_and_ilu1_apply_full(dev):

@@ -248,6 +248,38 @@ def test_spmv_transpose_follows_value_changes(dev):
         assert np.all(np.abs(y32.double().cpu().numpy() - ref) <= 2e-6 * _absbound(At, x) + 1e-30)
 
 
+def test_colblock_scattered_matrix(dev):
+    """A ragged matrix with columns scattered over an x larger than the L2
+    caches: AUTO runs the column-blocked form (ALBUS per column block, each
+    adding into y); alpha/beta, the FP32/BF16 value copies and value updates
+    all follow, against the oracle (FP64 1e-12 element-wise)."""
+    import torch
+    A = _o(la.gen.powerlaw(700_000, 8.0, 1.0, 3000, seed=6))
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    assert dA.resolve(la.SPMV_AUTO) == la.SPMV_COLBLOCK
+    assert dA.resolve(la.SPMV_ALBUS) == la.SPMV_ALBUS
+    x = la.gen.uniform(A.n, 7)
+    y0 = la.gen.uniform(A.n, 8)
+    xt = torch.tensor(x, device=dev)
+    for alpha, beta in [(1.0, 0.0), (2.5, -0.5)]:
+        yt = torch.tensor(y0, device=dev)
+        dA.spmv(xt, yt, alpha=alpha, beta=beta, alg=la.SPMV_AUTO)
+        ref = orc.spmv(A, x, y0, alpha=alpha, beta=beta)
+        scale = abs(alpha) * _absbound(A, x) + abs(beta) * np.abs(y0)
+        assert np.all(np.abs(yt.cpu().numpy() - ref) <= 1e-12 * scale + 1e-300), (alpha, beta)
+    for val in (la.F32, la.BF16):
+        yt = torch.zeros(A.n, dtype=torch.float32, device=dev)
+        dA.spmv(xt.float(), yt, alg=la.SPMV_AUTO, val=val, acc=la.F32)
+        ref_r = orc.spmv(_round_values(A, val), x)
+        assert np.all(np.abs(yt.double().cpu().numpy() - ref_r) <= ACC32_BOUND * _absbound(A, x))
+    av = np.random.default_rng(3).standard_normal(A.nnz)
+    dA.set_values(av)
+    yt = torch.zeros(A.n, dtype=torch.float64, device=dev)
+    dA.spmv(xt, yt, alg=la.SPMV_AUTO)
+    A2 = orc.CSR(A.n, A.ai, A.aj, av)
+    assert np.all(np.abs(yt.cpu().numpy() - orc.spmv(A2, x)) <= 1e-12 * _absbound(A2, x) + 1e-300)
+
+
 def test_value_grad(dev):
     import torch
     A = read_mtx("rdist1.mtx")
