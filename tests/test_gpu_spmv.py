@@ -248,6 +248,34 @@ def test_spmv_transpose_follows_value_changes(dev):
         assert np.all(np.abs(y32.double().cpu().numpy() - ref) <= 2e-6 * _absbound(At, x) + 1e-30)
 
 
+def test_documented_spmv_deviations(dev):
+    """Two documented parity exceptions, intended: (1) SegSumSPMV does not
+    special-case beta == 0 (spmv.hpp:171-173), so NaN in y survives in the
+    oracle's restatement; the device's SEGSUM follows SerialSPMV's rule
+    (beta == 0 never reads y, spmv.hpp:26). (2) ALBUSSPMV never visits
+    leading empty rows (spmv.hpp:213-218) and leaves them as they were; the
+    device writes them (0 for beta == 0, beta*y otherwise) like every other
+    row."""
+    import torch
+    D = random_sparse(200, 0.05, seed=11)
+    D[:4] = 0.0
+    A = from_dense(D)
+    x = la.gen.uniform(A.n, 3)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    y_nan = np.full(A.n, np.nan)
+    assert np.isnan(orc.spmv(A, x, y_nan, beta=0.0, alg="segsum", nthreads=4)).all()
+    yt = torch.tensor(y_nan, device=dev)
+    dA.spmv(torch.tensor(x, device=dev), yt, alg=la.SPMV_SEGSUM)
+    np.testing.assert_allclose(yt.cpu().numpy(), orc.spmv(A, x), rtol=1e-12, atol=1e-14)
+    y0 = la.gen.uniform(A.n, 4)
+    ref_albus = orc.spmv(A, x, y0, alpha=1.0, beta=0.5, alg="albus", nthreads=8)
+    np.testing.assert_array_equal(ref_albus[:4], y0[:4])        # never visited
+    yt = torch.tensor(y0, device=dev)
+    dA.spmv(torch.tensor(x, device=dev), yt, alpha=1.0, beta=0.5, alg=la.SPMV_ALBUS)
+    np.testing.assert_allclose(yt.cpu().numpy()[:4], 0.5 * y0[:4], rtol=0, atol=0)
+    np.testing.assert_allclose(yt.cpu().numpy()[4:], ref_albus[4:], rtol=1e-12, atol=1e-13)
+
+
 def test_colblock_scattered_matrix(dev):
     """A ragged matrix with columns scattered over an x larger than the L2
     caches: AUTO runs the column-blocked form (ALBUS per column block, each

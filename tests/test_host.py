@@ -309,3 +309,13 @@ def test_iluk_symbolic_threaded_bit_exact(name, make, level):
     np.testing.assert_array_equal(np.asarray(got_ai), np.asarray(ref_ai))
     np.testing.assert_array_equal(np.asarray(got_aj), np.asarray(ref_aj))
     np.testing.assert_array_equal(np.asarray(got_fu), np.asarray(ref_fu))
+
+
+def test_ilu0_product_is_textbook_not_level0_path():
+    """The product's ILU(0) equals the oracle's textbook factor bit for bit
+    (first_upper filled at level 0), not the reference's level-0 path; see
+    test_oracle.test_ilu0_reference_level0_path_differs_on_purpose."""
+    A = read_mtx("ex5.mtx")
+    Fo, fu, ok = orc.iluk(A, 0)
+    F = la.iluk(as_lib_csr(A), 0)
+    np.testing.assert_array_equal(F.av, Fo.av)

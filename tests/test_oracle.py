@@ -315,6 +315,31 @@ def test_ilu0_matches_A_on_pattern():
     assert np.max(np.abs(LU[mask] - Dd[mask])) <= 1e-9 * np.abs(Dd).max()
 
 
+def test_ilu0_reference_level0_path_differs_on_purpose():
+    """A documented parity exception. incomplete_lu_k at level 0 copies A's
+    pattern but never fills _firstUpperPos (incomplete_lu.cpp:164-174), so
+    numeric_factorize merges row i against the factor array from position 0
+    (:297-312) instead of from row k's first upper entry. Restated here by
+    handing the oracle's numeric phase a zero first_upper: that factor is
+    not pattern-exact, the textbook ILU(0) (what the product computes, with
+    first_upper filled) is. The product keeps the textbook factor."""
+    A = read_mtx("ex5.mtx")
+    F, fu, ok = orc.iluk(A, 0)
+    as_written = np.zeros(len(F.aj))
+    orc.lib().orc_iluk_numeric(A.n, A.base, A.ai, A.aj, A.av, F.ai, F.aj,
+                               np.zeros(A.n, np.int32), as_written)
+    Fw = orc.CSR(A.n, F.ai, F.aj, as_written)
+    Dd = to_dense(A)
+    mask = Dd != 0
+
+    def pattern_error(G):
+        L, D, U = orc.split_ldu(G)
+        LU = (to_dense(L) + np.eye(A.n)) @ (to_dense(U) + np.diag(D))
+        return np.max(np.abs(LU[mask] - Dd[mask])) / np.abs(Dd).max()
+    assert pattern_error(F) <= 1e-9
+    assert pattern_error(Fw) > 1e-3
+
+
 # ----------------------------------------------------------------- IC(k)
 
 @pytest.mark.parametrize("lvl", [0, 1, 2, 3])
