@@ -47,10 +47,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# the CPU baseline's thread placement: one thread per core,
-# packed; set before any OpenMP runtime starts
-os.environ.setdefault("OMP_PROC_BIND", "close")
-os.environ.setdefault("OMP_PLACES", "cores")
 
 METRIC = ("SpMV GB/s (%HBM peak); PCG time to 1e-10: mixed FP32/BF16 vs FP64, "
           "1 GPU")
@@ -154,8 +150,8 @@ def cpu_info():
     return _CPU_INFO
 
 
-# read at start-up: an OpenMP runtime with OMP_PROC_BIND binds the main thread
-# to one core once it starts, which would hide the process's CPU set
+# read at start-up: an OpenMP runtime started with OMP_PROC_BIND set binds the
+# main thread to one core, which would hide the process's CPU set
 _CPU_INFO = {}
 
 
@@ -185,10 +181,9 @@ def cpu_baseline(A, seconds=10.0, min_runs=10):
             "p10_p90_ms": [round(float(np.percentile(times, q)) * 1e3, 4) for q in (10, 90)],
             "runs": len(times),
             "sample": "%d single FP64 SpMVs after one warm-up (oracle ParallelSPMV "
-                      "restatement, OpenMP, %d threads, OMP_PROC_BIND=%s OMP_PLACES=%s) on the "
-                      "same RCM-ordered 128^3 7-point matrix, %.1f s; value = median" % (
-                          len(times), threads, os.environ.get("OMP_PROC_BIND"),
-                          os.environ.get("OMP_PLACES"), sum(times)),
+                      "restatement, OpenMP, %d threads) on the same RCM-ordered 128^3 "
+                      "7-point matrix, %.1f s; value = median" % (len(times), threads,
+                                                                 sum(times)),
             "cpu_model": ci["model"], "cpu_ident": ci["ident"],
             "logical_cpus": ci["logical_cpus"], "physical_cores": ci["physical_cores"]}
 
