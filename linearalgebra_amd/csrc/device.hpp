@@ -112,14 +112,16 @@ struct TrsvPlan {
   std::map<std::tuple<int, int, const void *, void *>, hipGraphExec_t> graphs;
   // tiled sync-free schedule (built when the pattern is recognised as a
   // grid): one workgroup per tile, its rows' dependencies inside the tile
-  // kept in LDS; layout arrays on the device, values in tiled order per
-  // precision (refreshed with the level-ordered values)
+  // kept in LDS. Rows are stored group by group, 64 slots per group (one
+  // per lane; slots past the group's rows are padding): the row id (-1 for
+  // padding), t_maxe 16-bit LDS indices of the values each row reads, and
+  // per precision its values and reciprocal diagonal (refreshed with the
+  // level-ordered values)
   bool tiled = false;
   int t_nparts = 0, t_ngroups = 0, t_mmax = 0, t_hmax = 0, t_maxe = 0, t_tau = 0;
   double t_remote = 0;
-  DevBuf t_gptr, t_grow, t_perm, t_code, t_rfptr, t_rfcol, t_rfslot;
-  DevBuf t_lsrc, t_dpos;  // tiled entry -> level-ordered entry (-1: padding); row -> level position
-  DevBuf t_bt;            // the right-hand side in tiled order (per solve)
+  DevBuf t_gptr, t_prow, t_pcode, t_rfptr, t_rfcol, t_rfslot;
+  DevBuf t_lsrc, t_dpos;  // slot entry -> level-ordered entry; slot -> level position (-1: padding)
   MultiValues t_val, t_dinv;
   unsigned t_wgs = 0;   // workgroups launched
   std::string t_why;    // why the tiled schedule was not built

@@ -199,29 +199,29 @@ __global__ __launch_bounds__(NT) void k_trsv_syncfree(
 // One workgroup per tile (see tiling.cpp): wavefront 0 solves the tile's
 // groups (at most 64 rows of one level each) in level order, one lane per
 // row, reading every x_j from LDS: its own results of the last kTileWindow
-// groups (`xl`) or the values of other tiles (`halo`). Wavefronts 1..NL
-// stage the groups ahead of it into a ring of LDS slots -- the rows' entry
-// codes and values, reciprocal diagonal, b and row index -- and poll the
-// other tiles' values each group reads first into `halo`; NL of them keep
-// NL groups' global loads in flight, so the staging keeps pace with the
-// solve. The dependency chain of a level is then a few LDS accesses (about
-// 0.1 us) instead of a hop through L2 between workgroups. Readiness between
-// the wavefronts is a per-slot group number (`s_ready`) and the count of
-// groups solved (`s_done`), both in LDS. A wavefront's LDS accesses are
-// carried out in the order it issues them, so the solving wavefront needs
-// no wait after its own writes (a compiler-only fence keeps the order in
-// the code); the staging wavefronts wait for their LDS writes before
-// publishing a slot. Every wait is bounded like the level-chunk kernel's;
-// a timeout sets *err.
-template <typename T> struct TileDt;
-template <> struct TileDt<double> { static constexpr la_dtype v = LA_F64; };
-template <> struct TileDt<float> { static constexpr la_dtype v = LA_F32; };
-template <> struct TileDt<bf16> { static constexpr la_dtype v = LA_BF16; };
-
+// groups or the values of other tiles (the halo), through one 16-bit LDS
+// index per entry. The rows' data is stored group by group, 64 slots per
+// group, and the solving wavefront loads it straight from HBM into
+// registers: each group's indices, values and 1/d, and b (gathered by row
+// id) kTileDepth groups ahead of the group it solves, the row ids twice as
+// far ahead. A group's dependency chain is then the x reads from LDS, the
+// products, and the LDS write and HBM store of the results (padding lanes
+// store nothing). Wavefronts 1..kTileHalo fetch the other tiles' values:
+// wavefront w takes the tile's groups w-1, w-1+kTileHalo, ..., polls the
+// values each group reads first (agent-scope loads of x, as in the
+// level-chunk kernel) into the halo, loading the next group's list while
+// it polls, and counts its finished groups in `s_prog[w-1]`; the solver
+// checks that count before a group's x reads. Halo slots are never reused
+// within a tile, so the fetching wavefronts run ahead freely. A wavefront's
+// LDS accesses are carried out in the order it issues them, so the solving
+// wavefront needs no wait after its own writes (a compiler-only fence keeps
+// the order in the code); the fetching wavefronts wait for their LDS writes
+// before counting a group. Every wait is bounded like the level-chunk
+// kernel's; a timeout sets *err.
 constexpr int kTileLanes = 64;
-constexpr int kTileRing = 24;     // ring slots (groups staged ahead)
-constexpr int kTileLoaders = 15;  // staging wavefronts per workgroup (16 in all)
-constexpr int kTileXl = kTileWindow * kTileLanes;  // LDS result slots; + 1 zero slot
+constexpr int kTileHalo = 7;    // halo wavefronts per workgroup (8 in all)
+constexpr int kTileDepth = 8;   // groups whose rows sit in registers
+constexpr int kTileXl = kTileWindow * kTileLanes;  // result slots; kTileXl is the zero slot
 
 __device__ __forceinline__ void lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup", "local");
@@ -233,176 +233,178 @@ __device__ __forceinline__ int lds_load(const int *p) {
 __device__ __forceinline__ void lds_store(int *p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-// spins until cond(*p) holds; false (with *err set) after kWaitTicks or
-// when another waiter has timed out. SLEEP: s_sleep units (64 clocks) per
-// poll -- the staging wavefronts poll slowly, so their reads do not queue
-// in the LDS ahead of the solving wavefront's
-template <int SLEEP, typename C>
-__device__ __forceinline__ bool spin_lds(const int *p, C cond, int *err) {
-  const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-  for (unsigned polls = 1;; ++polls) {
-    __builtin_amdgcn_s_sleep(SLEEP);
-    if (cond(lds_load(p))) return true;
-    if ((polls & 1023u) == 0 &&
-        (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-         __builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks)) {
-      __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return false;
-    }
+// N bytes from p (16-byte aligned for N >= 16, 8-byte for N == 8) into out
+typedef unsigned int tile_u2 __attribute__((ext_vector_type(2)));
+typedef unsigned int tile_u4 __attribute__((ext_vector_type(4)));
+template <int N>
+__device__ __forceinline__ void ld_bytes(const void *p, void *out) {
+  static_assert(N == 8 || N % 16 == 0, "load size");
+  if constexpr (N == 8) {
+    *reinterpret_cast<tile_u2 *>(out) =
+        __builtin_nontemporal_load(reinterpret_cast<const tile_u2 *>(p));
+  } else {
+#pragma unroll
+    for (int q = 0; q < N / 16; ++q)
+      reinterpret_cast<tile_u4 *>(out)[q] =
+          __builtin_nontemporal_load(reinterpret_cast<const tile_u4 *>(p) + q);
   }
 }
 
-template <int E, typename A>
-struct TileSlot {
-  // byte offsets inside one ring slot
-  static constexpr int code = 0;
-  static constexpr int val = code + E * kTileLanes * 4;
-  static constexpr int dv = val + E * kTileLanes * static_cast<int>(sizeof(A));
-  static constexpr int bv = dv + kTileLanes * static_cast<int>(sizeof(A));
-  static constexpr int pm = bv + kTileLanes * static_cast<int>(sizeof(A));
-  static constexpr int meta = pm + kTileLanes * 4;  // row count
-  static constexpr int bytes = (meta + 4 + 15) / 16 * 16;
+template <typename T> struct TileDt;
+template <> struct TileDt<double> { static constexpr la_dtype v = LA_F64; };
+template <> struct TileDt<float> { static constexpr la_dtype v = LA_F32; };
+template <> struct TileDt<bf16> { static constexpr la_dtype v = LA_BF16; };
+
+template <typename T> __device__ __forceinline__ float tile_val(T v) { return float(v); }
+template <> __device__ __forceinline__ float tile_val<bf16>(bf16 v) {
+  return __uint_as_float(static_cast<uint32_t>(v.bits) << 16);
+}
+__device__ __forceinline__ double tile_val(double v) { return v; }
+
+// one pipeline stage: a group's slot for this lane
+template <int E, typename T>
+struct TileStage {
+  alignas(16) uint16_t code[E];
+  alignas(16) T val[E];
+  T dinv;
 };
 
-inline size_t tile_lds_bytes(int h_max, size_t vsize, int slot_bytes) {
-  const size_t head = ((static_cast<size_t>(kTileXl) + 1 + h_max) * vsize + 15) / 16 * 16;
-  return head + static_cast<size_t>(kTileRing) * slot_bytes;
-}
-
-template <int E, int NL, typename T, typename V, typename A, bool DIAG>
-__global__ __launch_bounds__(kTileLanes * (1 + NL)) void k_trsv_tiled(
-    int nparts, const int *__restrict__ gptr, const int *__restrict__ grow,
-    const int *__restrict__ tperm, const int *__restrict__ tcode, const T *__restrict__ tval,
-    const T *__restrict__ tdinv, const int *__restrict__ rfptr, const int *__restrict__ rfcol,
-    const int *__restrict__ rfslot, int h_max, const V *__restrict__ bt, V *x, unsigned *ctl,
-    int *err) {
-  using S = TileSlot<E, A>;
+template <int E, typename T, typename V, typename A, bool DIAG>
+__global__ __launch_bounds__(kTileLanes * (1 + kTileHalo)) void k_trsv_tiled(
+    int nparts, const int *__restrict__ gptr, const int *__restrict__ prow,
+    const uint16_t *__restrict__ pcode, const T *__restrict__ pval, const T *__restrict__ pdinv,
+    const int *__restrict__ rfptr, const int *__restrict__ rfcol, const int *__restrict__ rfslot,
+    const V *__restrict__ b, V *x, unsigned *ctl, int *err) {
+  // pipeline depth: rows of 8 entries hold twice the registers
+  constexpr int D = E <= 4 ? kTileDepth : kTileDepth * 3 / 4, H = kTileHalo;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int s_ready[kTileRing];
-  __shared__ int s_done, s_part;
-  V *xl = reinterpret_cast<V *>(smem);
-  V *halo = xl + (kTileXl + 1);
-  unsigned char *ring = smem + ((static_cast<size_t>(kTileXl) + 1 + h_max) * sizeof(V) + 15) / 16 * 16;
+  __shared__ int s_prog[H];
+  __shared__ int s_part, s_late;
+  V *xs = reinterpret_cast<V *>(smem);  // window, zero slot, halo
   const int wave = static_cast<int>(threadIdx.x) / kTileLanes;
   const int lane = static_cast<int>(threadIdx.x) % kTileLanes;
   for (;;) {
     if (threadIdx.x == 0) {
       s_part = static_cast<int>(atomicAdd(&ctl[0], 1u));
-      s_done = 0;
-      xl[kTileXl] = V(0);  // the padding slot
+      s_late = 0;
+      xs[kTileXl] = V(0);
     }
-    if (threadIdx.x < kTileRing) s_ready[threadIdx.x] = -1;
+    if (threadIdx.x < H) s_prog[threadIdx.x] = 0;
     __syncthreads();
     const int p = s_part;
     if (p >= nparts) break;
-    const int g0 = gptr[p], g1 = gptr[p + 1];
-    if (wave == 0) {
-      __builtin_amdgcn_s_setprio(3);  // issue priority over the staging wavefronts
-      // the slot of group i into registers: codes (padding for lanes past
-      // the group's rows, whose slot data is stale), values, b, 1/d, row
-      int c[E], nr = 0, pm = 0;
-      A a[E], bv = A(0), dv = A(1);
-      auto read_slot = [&](int i) {
-        const unsigned char *sb = ring + (i % kTileRing) * S::bytes;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-          c[e] = reinterpret_cast<const int *>(sb + S::code)[e * kTileLanes + lane];
-          a[e] = reinterpret_cast<const A *>(sb + S::val)[e * kTileLanes + lane];
-        }
-        bv = reinterpret_cast<const A *>(sb + S::bv)[lane];
-        if constexpr (DIAG) dv = reinterpret_cast<const A *>(sb + S::dv)[lane];
-        pm = reinterpret_cast<const int *>(sb + S::pm)[lane];
-        nr = *reinterpret_cast<const int *>(sb + S::meta);
+    const int g0 = gptr[p], ng = gptr[p + 1] - g0;
+    if (wave == 0 && ng > 0) {
+      __builtin_amdgcn_s_setprio(3);  // issue priority over the halo wavefronts
+      // group j's slot of this lane (j clamped: loads past the last group
+      // repeat it, so every iteration issues the same loads)
+      auto slot_of = [&](int j) {
+        return static_cast<int64_t>(g0 + min(j, ng - 1)) * kTileLanes + lane;
       };
-      auto wait_slot = [&](int i) {
-        if (lds_load(&s_ready[i % kTileRing]) != i) {
-          spin_lds<0>(&s_ready[i % kTileRing], [i](int v) { return v == i; }, err);
-          lds_fence();
-        }
-        code_order();
-        read_slot(i);
+      // row ids 2D groups ahead, the rows' data and b (gathered by row id)
+      // D groups ahead
+      int rid[2 * D];
+      TileStage<E, T> st[D];
+      V bv[D];
+      auto load_rows = [&](int k, int j, int rk) {  // rk: the ring slot of j's row id
+        const int64_t sl = slot_of(j);
+        ld_bytes<E * 2>(pcode + sl * E, st[k].code);
+        ld_bytes<E * static_cast<int>(sizeof(T))>(pval + sl * E, st[k].val);
+        if constexpr (DIAG) st[k].dinv = pdinv[sl];
+        bv[k] = b[max(rid[rk], 0)];
       };
-      const int ng = g1 - g0;
-      if (ng > 0) wait_slot(0);
-      for (int i = 0; i < ng; ++i) {
-        if (lane >= nr) {
+      // prime in the order the loop keeps, so that the loads in flight
+      // when the loop starts are as old as in any later pass (the
+      // compiler's count of loads allowed in flight then stays high;
+      // sched_barrier keeps the scheduler from reordering them)
 #pragma unroll
-          for (int e = 0; e < E; ++e) c[e] = kTileXl;
-        }
-        V xv[E];
+      for (int k = 0; k < 2 * D; ++k) rid[k] = __builtin_nontemporal_load(prow + slot_of(k));
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int e = 0; e < E; ++e) xv[e] = c[e] >= 0 ? xl[c[e]] : halo[-c[e] - 1];
-        const A ai0 = a[0];
-        A acc = ai0 * A(xv[0]);
-#pragma unroll
-        for (int e = 1; e < E; ++e) acc += a[e] * A(xv[e]);
-        A v = bv - acc;
-        if constexpr (DIAG) v = v * dv;
-        const V out = V(v);
-        const bool live = lane < nr;
-        const int opm = pm;
-        // the next group's slot, read while this group's results are
-        // written (a slot read before its readiness is seen is read again)
-        int rdy = -1;
-        if (i + 1 < ng) {
-          rdy = lds_load(&s_ready[(i + 1) % kTileRing]);
-          code_order();
-          read_slot(i + 1);
-        }
-        if (live) {
-          xl[(i % kTileWindow) * kTileLanes + lane] = out;
-          publish(x + opm, out);
-        }
-        code_order();
-        if (lane == 0) lds_store(&s_done, i + 1);
-        if (i + 1 < ng && rdy != i + 1) wait_slot(i + 1);
+      for (int k = 0; k < D; ++k) {
+        load_rows(k, k, k);
+        __builtin_amdgcn_sched_barrier(0);
       }
-    } else {
-      for (int g = g0 + wave - 1; g < g1; g += NL) {
-        const int i = g - g0;
-        unsigned char *sb = ring + (i % kTileRing) * S::bytes;
-        // the slot is free once group i - kTileRing is solved
-        if (i >= kTileRing && lds_load(&s_done) < i - kTileRing + 1)
-          spin_lds<4>(&s_done, [i](int v) { return v >= i - kTileRing + 1; }, err);
-        lds_fence();
-        const int r0 = grow[g], nr = grow[g + 1] - r0;
-        const int f0 = rfptr[g], f1 = rfptr[g + 1];
-        // the other tiles' values this group reads first (one per lane in
-        // the common case), loaded alongside the rows' data
-        V hv = V(0);
-        if (f0 + lane < f1) hv = load_ready(x + rfcol[f0 + lane]);
-        if (lane < nr) {
-          const int k = r0 + lane;
-          int *code = reinterpret_cast<int *>(sb + S::code);
-          A *val = reinterpret_cast<A *>(sb + S::val);
-          int cc[E];
-          A vv[E];
+      int rdy = lds_load(&s_prog[0]);
+      for (int i0 = 0; i0 < ng; i0 += 2 * D) {
 #pragma unroll
-          for (int e = 0; e < E; ++e) {
-            cc[e] = tcode[static_cast<int64_t>(k) * E + e];
-            vv[e] = A(ldt(tval, static_cast<int64_t>(k) * E + e));
+        for (int k2 = 0; k2 < 2 * D; ++k2) {
+          // groups past the last one (at most 2D - 1) repeat its work
+          // without publishing: a loop without early exits keeps the
+          // compiler's count of loads in flight across iterations
+          const int i = i0 + k2, k = k2 % D;
+          const bool real = i < ng;
+          // the halo values of group i are in LDS once its fetcher has
+          // counted it
+          if (real && rdy <= i / H) {
+            // no global access on this path: the loads in flight are
+            // counted in issue order, and a global access here would make
+            // the compiler wait for all of them
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            for (unsigned polls = 1; lds_load(&s_prog[i % H]) <= i / H; ++polls) {
+              __builtin_amdgcn_s_sleep(0);
+              if ((polls & 1023u) == 0 && __builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                s_late = 1;
+                break;
+              }
+            }
+            lds_fence();
           }
-          A d = A(1);
-          if constexpr (DIAG) d = A(ldt(tdinv, k));
-          const int r = tperm[k];
-          const A bb = A(bt[k]);
+          code_order();
+          V xv[E];
 #pragma unroll
-          for (int e = 0; e < E; ++e) {
-            code[e * kTileLanes + lane] = cc[e];
-            val[e * kTileLanes + lane] = vv[e];
-          }
-          reinterpret_cast<A *>(sb + S::dv)[lane] = d;
-          reinterpret_cast<int *>(sb + S::pm)[lane] = r;
-          reinterpret_cast<A *>(sb + S::bv)[lane] = bb;
+          for (int e = 0; e < E; ++e) xv[e] = xs[st[k].code[e]];
+          A acc = A(tile_val(st[k].val[0])) * A(xv[0]);
+#pragma unroll
+          for (int e = 1; e < E; ++e) acc += A(tile_val(st[k].val[e])) * A(xv[e]);
+          A v = A(bv[k]) - acc;
+          if constexpr (DIAG) v = v * A(tile_val(st[k].dinv));
+          const V out = V(v);
+          const int r = rid[k2];
+          xs[(i % kTileWindow) * kTileLanes + lane] = out;
+          if (real && r >= 0) publish(x + r, out);
+          code_order();
+          rdy = lds_load(&s_prog[(i + 1) % H]);
+          // refill: the row id of group i + 2D, the rows of group i + D
+          rid[k2] = __builtin_nontemporal_load(prow + slot_of(i + 2 * D));
+          load_rows(k, i + D, (k2 + D) % (2 * D));
         }
-        if (lane == 0) *reinterpret_cast<int *>(sb + S::meta) = nr;
+      }
+    } else if (wave > 0) {
+      // this wavefront's groups j = hw, hw + H, ... (hw = wave - 1): the
+      // next group's list (its halo sources and slots) is loaded while the
+      // current group's values are polled
+      const int hw = wave - 1;
+      int j = hw, f0 = 0, f1 = 0, col = 0, slot = 0;
+      if (j < ng) {
+        f0 = rfptr[g0 + j];
+        f1 = rfptr[g0 + j + 1];
+        if (f0 + lane < f1) {
+          col = rfcol[f0 + lane];
+          slot = rfslot[f0 + lane];
+        }
+      }
+      for (; j < ng; j += H) {
+        V xv = V(0);
+        if (f0 + lane < f1) xv = load_ready(x + col);
+        const int jn = j + H;
+        int nf0 = 0, nf1 = 0, ncol = 0, nslot = 0;
+        if (jn < ng) {
+          nf0 = rfptr[g0 + jn];
+          nf1 = rfptr[g0 + jn + 1];
+          if (nf0 + lane < nf1) {
+            ncol = rfcol[nf0 + lane];
+            nslot = rfslot[nf0 + lane];
+          }
+        }
         for (int t = f0 + lane; t < f1; t += kTileLanes) {
-          V xv = t == f0 + lane ? hv : load_ready(x + rfcol[t]);
+          const int c = t == f0 + lane ? col : rfcol[t];
+          if (t != f0 + lane) xv = load_ready(x + c);
           if (is_sentinel(xv)) {
             const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
             for (unsigned polls = 1; is_sentinel(xv); ++polls) {
               __builtin_amdgcn_s_sleep(1);
-              xv = load_ready(x + rfcol[t]);
+              xv = load_ready(x + c);
               if ((polls & 255u) == 0 &&
                   (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
                    __builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks)) {
@@ -412,14 +414,21 @@ __global__ __launch_bounds__(kTileLanes * (1 + NL)) void k_trsv_tiled(
               }
             }
           }
-          halo[rfslot[t]] = xv;
+          xs[t == f0 + lane ? slot : rfslot[t]] = xv;
         }
         lds_fence();
         __builtin_amdgcn_wave_barrier();
-        if (lane == 0) lds_store(&s_ready[i % kTileRing], i);
+        if (lane == 0) lds_store(&s_prog[hw], j / H + 1);
+        f0 = nf0;
+        f1 = nf1;
+        col = ncol;
+        slot = nslot;
       }
     }
     __syncthreads();
+    // a solver wait that ran out of time is reported here
+    if (threadIdx.x == 0 && s_late)
+      __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -467,25 +476,26 @@ void sf_launch(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x, hipSt
                      P.nchunks, P.chunk_start.as<int>(), rp, cl, val, diag, pm, b, x, ctl, err);
 }
 
-// x <- sentinel, bt <- b in tiled order, control words reset (the tiled
-// schedule's stand-in for k_fill_sentinel)
+// x <- sentinel, control words reset (the tiled schedule's stand-in for
+// k_fill_sentinel)
 template <typename V>
-__global__ void k_tile_prep(int64_t n, const int *__restrict__ tperm, const V *__restrict__ b,
-                            V *bt, V *x, unsigned *ctl) {
+__global__ void k_tile_prep(int64_t n, V *x, unsigned *ctl) {
   using U = typename Sentinel<V>::U;
   U *xb = reinterpret_cast<U *>(x);
   for (int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; t < n;
-       t += static_cast<int64_t>(gridDim.x) * kB) {
-    bt[t] = b[tperm[t]];
+       t += static_cast<int64_t>(gridDim.x) * kB)
     xb[t] = Sentinel<V>::bits;
-  }
   if (blockIdx.x == 0 && threadIdx.x < 16) ctl[threadIdx.x] = 0;
+}
+
+inline size_t tile_lds_bytes(int h_max, size_t vsize) {
+  return (static_cast<size_t>(kTileXl) + 1 + h_max) * vsize;
 }
 
 template <int E, typename T, typename V, typename A, bool DIAG>
 void tiled_launch(TrsvPlan &P, const V *b, V *x, hipStream_t s) {
-  auto kern = k_trsv_tiled<E, kTileLoaders, T, V, A, DIAG>;
-  const size_t lds = tile_lds_bytes(P.t_hmax, sizeof(V), TileSlot<E, A>::bytes);
+  auto kern = k_trsv_tiled<E, T, V, A, DIAG>;
+  const size_t lds = tile_lds_bytes(P.t_hmax, sizeof(V));
   static size_t lds_set = 0;  // per instantiation: the largest size allowed so far
   if (lds > lds_set) {
     LA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
@@ -493,18 +503,15 @@ void tiled_launch(TrsvPlan &P, const V *b, V *x, hipStream_t s) {
                                static_cast<int>(lds)));
     lds_set = lds;
   }
-  const T *tv = P.nnz ? static_cast<const T *>(P.t_val.get(TileDt<T>::v)) : nullptr;
+  const T *tv = static_cast<const T *>(P.t_val.get(TileDt<T>::v));
   const T *td = DIAG ? static_cast<const T *>(P.t_dinv.get(TileDt<T>::v)) : nullptr;
-  V *bt = P.t_bt.as<V>();
   const unsigned fg = static_cast<unsigned>(std::min<int64_t>(4096, (P.n + kB - 1) / kB));
   hipLaunchKernelGGL(k_tile_prep<V>, dim3(std::max(fg, 1u)), dim3(kB), 0, s,
-                     static_cast<int64_t>(P.n), P.t_perm.as<int>(), b, bt, x,
-                     P.counter.as<unsigned>());
-  hipLaunchKernelGGL(kern, dim3(P.t_wgs), dim3(kTileLanes * (1 + kTileLoaders)), lds, s,
-                     P.t_nparts, P.t_gptr.as<int>(), P.t_grow.as<int>(), P.t_perm.as<int>(),
-                     P.t_code.as<int>(), tv, td, P.t_rfptr.as<int>(), P.t_rfcol.as<int>(),
-                     P.t_rfslot.as<int>(), P.t_hmax, bt, x, P.counter.as<unsigned>(),
-                     P.err_flag.as<int>());
+                     static_cast<int64_t>(P.n), x, P.counter.as<unsigned>());
+  hipLaunchKernelGGL(kern, dim3(P.t_wgs), dim3(kTileLanes * (1 + kTileHalo)), lds, s,
+                     P.t_nparts, P.t_gptr.as<int>(), P.t_prow.as<int>(),
+                     P.t_pcode.as<uint16_t>(), tv, td, P.t_rfptr.as<int>(), P.t_rfcol.as<int>(),
+                     P.t_rfslot.as<int>(), b, x, P.counter.as<unsigned>(), P.err_flag.as<int>());
 }
 
 template <typename T, typename V, typename A, bool DIAG>
@@ -708,7 +715,7 @@ void jacobi_solve(TrsvPlan &T, la_dtype val, la_dtype vec, const void *b, void *
   jac_scale(n, val, vec, d, tmp, x, s);
 }
 
-// tiled order <- level order: the plan's values (entry source index, -1 for
+// slot order <- level order: the plan's values (entry source index, -1 for
 // padding) and reciprocal diagonal
 __global__ void k_tile_gather(int64_t m, const int *__restrict__ src, const double *__restrict__ f,
                               double *out) {
@@ -718,7 +725,8 @@ __global__ void k_tile_gather(int64_t m, const int *__restrict__ src, const doub
 
 void tiled_refresh(TrsvPlan &T, hipStream_t s) {
   if (!T.tiled) return;
-  const int64_t m = static_cast<int64_t>(T.n) * T.t_maxe;
+  const int64_t slots = static_cast<int64_t>(T.t_ngroups) * kTileLanes;
+  const int64_t m = slots * T.t_maxe;
   if (T.nnz) {
     double *tv = T.t_val.v[LA_F64].as<double>();
     hipLaunchKernelGGL(k_tile_gather, dim3(static_cast<unsigned>((m + 255) / 256)), dim3(256), 0,
@@ -728,80 +736,86 @@ void tiled_refresh(TrsvPlan &T, hipStream_t s) {
   }
   if (!T.unit) {
     double *td = T.t_dinv.v[LA_F64].as<double>();
-    hipLaunchKernelGGL(k_tile_gather, dim3(static_cast<unsigned>((T.n + 255) / 256)), dim3(256),
-                       0, s, static_cast<int64_t>(T.n), T.t_dpos.as<int>(),
+    hipLaunchKernelGGL(k_tile_gather, dim3(static_cast<unsigned>((slots + 255) / 256)),
+                       dim3(256), 0, s, slots, T.t_dpos.as<int>(),
                        T.diag.v[LA_F64].as<double>(), td);
     LA_HIP(hipGetLastError());
-    T.t_dinv.set_from_device_f64(td, T.n, s);
+    T.t_dinv.set_from_device_f64(td, slots, s);
   }
 }
 
 // The tiled schedule when the triangle's pattern is recognised as a grid
-// and a tile's rows, faces and staging ring fit one workgroup's LDS; one
-// workgroup per tile, at most one per CU, so every tile is resident.
+// and a tile's window and halo fit one workgroup's LDS; one workgroup per
+// tile, at most one per CU, so every tile is resident.
 void build_tiled(TrsvPlan &P, const std::vector<int> &iperm, const std::vector<int> &lrp,
                  hipStream_t s) {
   const int n = P.n;
+  if (P.nnz == 0) {
+    P.t_why = "no entries";
+    return;
+  }
   std::vector<int> level(static_cast<size_t>(n)), lpos(static_cast<size_t>(n));
   for (int l = 0; l < P.nlevels; ++l)
     for (int k = P.level_ptr[l]; k < P.level_ptr[l + 1]; ++k) {
       level[iperm[k]] = l;
       lpos[iperm[k]] = k;
     }
-  int maxe = 0;
-  for (int i = 0; i < n; ++i) maxe = std::max(maxe, P.h_ai[i + 1] - P.h_ai[i]);
-  const int slot = maxe <= 4 ? TileSlot<4, double>::bytes : TileSlot<kTileMaxEntries, double>::bytes;
   int dev = 0, ncu = 0, budget = 0;
   LA_HIP(hipGetDevice(&dev));
   LA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   LA_HIP(hipDeviceGetAttribute(&budget, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  const int64_t fixed = static_cast<int64_t>(tile_lds_bytes(0, sizeof(double), slot));
-  const int64_t avail = static_cast<int64_t>(budget) - fixed - 64;
-  if (avail < 1024 || ncu < 1) {
+  // halo slots: what LDS holds beyond the window, and what a 16-bit index
+  // reaches
+  const int64_t avail = static_cast<int64_t>(budget) - static_cast<int64_t>(tile_lds_bytes(0, 8));
+  const int max_halo = static_cast<int>(std::min<int64_t>(avail / 8, 65535 - kTileXl - 1));
+  if (max_halo < 128 || ncu < 1) {
     P.t_why = "LDS too small for the tiled schedule";
     return;
   }
-  const int max_halo = static_cast<int>(avail / 8);
   TileLayout L;
   if (!build_tile_layout(n, P.h_ai.data(), P.h_aj.data(), level.data(), kTileLanes, ncu, n,
                          max_halo, L, P.t_why))
     return;
-  if (tile_lds_bytes(L.h_max, sizeof(double), slot) > static_cast<size_t>(budget)) {
-    P.t_why = "tile does not fit LDS";
-    return;
-  }
   const int E = L.maxe;
-  const int64_t m = static_cast<int64_t>(n) * E;
-  std::vector<int> lsrc(static_cast<size_t>(m)), dpos(static_cast<size_t>(n));
-  std::vector<double> tv(static_cast<size_t>(m)), td(P.unit ? 0 : static_cast<size_t>(n));
-  for (int k = 0; k < n; ++k) {
-    const int r = L.tperm[k];
-    dpos[k] = lpos[r];
-    if (!P.unit) td[k] = 1.0 / P.h_diag[r];
-    for (int e = 0; e < E; ++e) {
-      const int q = L.tsrc[static_cast<size_t>(k) * E + e];
-      lsrc[static_cast<size_t>(k) * E + e] = q >= 0 ? lrp[lpos[r]] + (q - P.h_ai[r]) : -1;
-      tv[static_cast<size_t>(k) * E + e] = q >= 0 ? P.h_av[q] : 0.0;
+  const int G = L.ngroups;
+  const int64_t slots = static_cast<int64_t>(G) * kTileLanes, m = slots * E;
+  std::vector<int> prow(static_cast<size_t>(slots), -1), dpos(static_cast<size_t>(slots), -1);
+  std::vector<int> lsrc(static_cast<size_t>(m), -1);
+  std::vector<uint16_t> pcode(static_cast<size_t>(m), static_cast<uint16_t>(kTileXl));
+  std::vector<double> tv(static_cast<size_t>(m), 0.0), td(static_cast<size_t>(slots), 0.0);
+  for (int g = 0; g < G; ++g)
+    for (int kk = L.grp_row[g]; kk < L.grp_row[g + 1]; ++kk) {
+      const int64_t sl = static_cast<int64_t>(g) * kTileLanes + (kk - L.grp_row[g]);
+      const int r = L.tperm[kk];
+      prow[sl] = r;
+      dpos[sl] = lpos[r];
+      if (!P.unit) td[sl] = 1.0 / P.h_diag[r];
+      for (int e = 0; e < E; ++e) {
+        const size_t te = static_cast<size_t>(kk) * E + e, pe = static_cast<size_t>(sl) * E + e;
+        const int c = L.tcode[te], q = L.tsrc[te];
+        pcode[pe] = static_cast<uint16_t>(c >= 0 ? c : kTileXl + 1 + (-c - 1));
+        lsrc[pe] = q >= 0 ? lrp[lpos[r]] + (q - P.h_ai[r]) : -1;
+        tv[pe] = q >= 0 ? P.h_av[q] : 0.0;
+      }
     }
-  }
-  auto up = [&](DevBuf &d, const std::vector<int> &h) {
-    d.alloc(sizeof(int) * std::max<size_t>(h.size(), 1));
-    d.upload(h.data(), sizeof(int) * h.size(), s);
+  std::vector<int> rfslot(L.rf_slot.size());
+  for (size_t t = 0; t < rfslot.size(); ++t) rfslot[t] = kTileXl + 1 + L.rf_slot[t];
+  auto up = [&](DevBuf &d, const void *h, size_t bytes) {
+    d.alloc(std::max<size_t>(bytes, 4));
+    d.upload(h, bytes, s);
   };
-  up(P.t_gptr, L.part_gptr);
-  up(P.t_grow, L.grp_row);
-  up(P.t_perm, L.tperm);
-  up(P.t_code, L.tcode);
-  up(P.t_rfptr, L.rf_ptr);
-  up(P.t_rfcol, L.rf_col);
-  up(P.t_rfslot, L.rf_slot);
-  up(P.t_lsrc, lsrc);
-  up(P.t_dpos, dpos);
-  P.t_bt.alloc(sizeof(double) * static_cast<size_t>(n));
-  if (P.nnz) P.t_val.set_from_host(tv.data(), m, s);
-  if (!P.unit) P.t_dinv.set_from_host(td.data(), n, s);
+  up(P.t_gptr, L.part_gptr.data(), sizeof(int) * L.part_gptr.size());
+  up(P.t_prow, prow.data(), sizeof(int) * prow.size());
+  up(P.t_pcode, pcode.data(), sizeof(uint16_t) * pcode.size());
+  up(P.t_rfptr, L.rf_ptr.data(), sizeof(int) * L.rf_ptr.size());
+  up(P.t_rfcol, L.rf_col.data(), sizeof(int) * L.rf_col.size());
+  up(P.t_rfslot, rfslot.data(), sizeof(int) * rfslot.size());
+  up(P.t_lsrc, lsrc.data(), sizeof(int) * lsrc.size());
+  up(P.t_dpos, dpos.data(), sizeof(int) * dpos.size());
+  P.t_val.set_from_host(tv.data(), m, s);
+  if (!P.unit) P.t_dinv.set_from_host(td.data(), slots, s);
   P.t_nparts = L.nparts;
-  P.t_ngroups = L.ngroups;
+  P.t_ngroups = G;
   P.t_mmax = L.m_max;
   P.t_hmax = L.h_max;
   P.t_maxe = E;
