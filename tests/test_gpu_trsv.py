@@ -283,6 +283,55 @@ def test_tiled_schedule_ilu0_unit_lower(dev, name, make):
     assert _close(x.cpu().numpy(), orc.backward_sub(U, D, b), 1e-12)
 
 
+def _star13(nx, ny, nz):
+    """3-D 13-point star stencil (neighbours at distance 1 and 2 along each
+    axis), diagonally dominant: its IC(0) triangle has rows of up to 6
+    entries, the tiled kernel's 8-entry form."""
+    n = nx * ny * nz
+    idx = np.arange(n).reshape(nz, ny, nx)
+    rows, cols, vals = [idx.ravel()], [idx.ravel()], [np.full(n, 10.0)]
+    for ax in range(3):
+        for d, w in ((1, -1.0), (2, -0.5)):
+            sa, sb = [slice(None)] * 3, [slice(None)] * 3
+            sa[ax], sb[ax] = slice(0, idx.shape[ax] - d), slice(d, None)
+            a, b = idx[tuple(sa)].ravel(), idx[tuple(sb)].ravel()
+            rows += [a, b]
+            cols += [b, a]
+            vals += [np.full(a.size, w)] * 2
+    r, c, v = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    ai = np.concatenate([[0], np.cumsum(np.bincount(r, minlength=n))]).astype(np.int32)
+    return orc.CSR(n, ai, c.astype(np.int32), v)
+
+
+@pytest.mark.parametrize("dims", [(24, 20, 16), (17, 13, 11)])
+def test_tiled_schedule_wide_rows(dev, dims):
+    """Rows of 5-8 entries run the tiled kernel's 8-entry form (a shallower
+    register pipeline); both triangles of the IC(0) apply match the oracle
+    in FP64 and the FP32 bound, and repeat bitwise."""
+    import torch
+    A = _star13(*dims)
+    R, ok = orc.icc(orc.split_triangle(A, True), 0)
+    assert ok
+    P = la.Preconditioner(as_lib_csr(R), "ic")
+    tl, tu = P.tiles()
+    print("case star13", dims, "tiles", tl, tu, flush=True)
+    assert tl > 1 and tu > 1
+    b = la.gen.uniform(A.n, 15)
+    ref = orc.prec_apply(A, "ic", R, b)
+    bt = torch.tensor(b, device=dev)
+    z = torch.zeros_like(bt)
+    P.apply(bt, z, mode=la.TRSV_SYNCFREE)
+    assert _close(z.cpu().numpy(), ref, 1e-12)
+    z2 = torch.zeros_like(bt)
+    P.apply(bt, z2, mode=la.TRSV_SYNCFREE)
+    assert torch.equal(z2, z)
+    z32 = torch.zeros(A.n, dtype=torch.float32, device=dev)
+    P.apply(bt.float(), z32, mode=la.TRSV_SYNCFREE, val=la.F32)
+    assert np.linalg.norm(z32.double().cpu().numpy() - ref) <= 1e-4 * np.linalg.norm(ref)
+
+
 def test_tiled_schedule_off_and_non_grid(dev, monkeypatch):
     """LAAMD_SF_TILED=0 at plan creation keeps the level-chunk schedule;
     a pattern that is not a grid keeps it too; both solve exactly."""
