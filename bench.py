@@ -638,13 +638,15 @@ def main():
     y = torch.zeros(n, dtype=torch.float64, device=dev)
 
     if args.pmc_probe:
-        for i in range(6):
-            mats[i % 3].spmv(x, y, alg=la.SPMV_AUTO)
+        for alg in (la.SPMV_AUTO, la.SPMV_ALBUS, la.SPMV_ROW):
+            for i in range(6):
+                mats[i % 3].spmv(x, y, alg=alg)
         big = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
         dst = torch.empty_like(big)
         dst.copy_(big)
         torch.cuda.synchronize()
-        print(json.dumps({"probe": "spmv x6 + copy 1GiB", "spmv_bytes": spmv_bytes(n, nnz)}))
+        print(json.dumps({"probe": "spmv auto (sliced ELL) x6, albus x6, row x6 + copy 1GiB",
+                          "spmv_bytes": spmv_bytes(n, nnz)}))
         return
 
     # parity check of the timed kernel against the FP64 oracle (cheap)
