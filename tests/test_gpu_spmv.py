@@ -279,8 +279,10 @@ def test_documented_spmv_deviations(dev):
 def test_colblock_scattered_matrix(dev):
     """A ragged matrix with columns scattered over an x larger than the L2
     caches: AUTO runs the column-blocked form (ALBUS per column block, each
-    adding into y); alpha/beta, the FP32/BF16 value copies and value updates
-    all follow, against the oracle (FP64 1e-12 element-wise)."""
+    adding into y); alpha/beta (beta = 1 makes every pass accumulate), the
+    FP32/BF16 value
+    copies and value updates all follow, against the oracle (FP64 1e-12
+    element-wise)."""
     import torch
     A = _o(la.gen.powerlaw(700_000, 8.0, 1.0, 3000, seed=6))
     dA = la.DeviceMatrix(as_lib_csr(A))
@@ -289,7 +291,7 @@ def test_colblock_scattered_matrix(dev):
     x = la.gen.uniform(A.n, 7)
     y0 = la.gen.uniform(A.n, 8)
     xt = torch.tensor(x, device=dev)
-    for alpha, beta in [(1.0, 0.0), (2.5, -0.5)]:
+    for alpha, beta in [(1.0, 0.0), (2.5, -0.5), (-1.0, 1.0)]:
         yt = torch.tensor(y0, device=dev)
         dA.spmv(xt, yt, alpha=alpha, beta=beta, alg=la.SPMV_AUTO)
         ref = orc.spmv(A, x, y0, alpha=alpha, beta=beta)
