@@ -155,13 +155,16 @@ def cpu_info():
 _CPU_INFO = {}
 
 
-def cpu_baseline(A, seconds=10.0, min_runs=10):
-    """The oracle's OpenMP ParallelSPMV restatement on this host's cores:
-    one warm-up, then single SpMVs timed one by one until `seconds` have
-    passed (at least `min_runs`); the median run is reported."""
+def cpu_child(path, seconds, min_runs=10):
+    """The CPU baseline's timing, in a process of its own started with
+    OMP_PROC_BIND=close and OMP_PLACES=cores: the oracle's OpenMP
+    ParallelSPMV restatement on the matrix saved at `path`; one warm-up,
+    then single SpMVs timed one by one until `seconds` have passed (at
+    least `min_runs`). Prints the times as one JSON line."""
     from oracle import oracle as orc
-    Ao = orc.CSR(A.n, A.ai, A.aj, A.av)
-    x = np.random.default_rng(1).uniform(-1, 1, A.n)
+    d = np.load(path)
+    Ao = orc.CSR(int(d["n"]), d["ai"], d["aj"], d["av"])
+    x = np.random.default_rng(1).uniform(-1, 1, Ao.n)
     threads = orc.max_threads()
     orc.spmv(Ao, x, alg="parallel", nthreads=threads)  # warm-up
     times, t_start = [], time.perf_counter()
@@ -171,9 +174,32 @@ def cpu_baseline(A, seconds=10.0, min_runs=10):
         times.append(time.perf_counter() - t0)
         if len(times) >= 100000:
             break
+    print(json.dumps({"times": times, "threads": threads}))
+
+
+def cpu_baseline(A, seconds=10.0):
+    """The oracle's OpenMP ParallelSPMV restatement on this host's cores,
+    timed in a child process with its threads packed onto cores
+    (OMP_PROC_BIND=close, OMP_PLACES=cores; this process's own threads stay
+    unbound); the median run is reported."""
+    import subprocess
+    import tempfile
+    ci = cpu_info()
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        path = os.path.join(tmp, "a.npz")
+        np.savez(path, n=np.int64(A.n), ai=np.asarray(A.ai), aj=np.asarray(A.aj),
+                 av=np.asarray(A.av))
+        # the thread count the environment sets (the host's CPU share), else
+        # one per logical CPU this process may use
+        env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+        env.setdefault("OMP_NUM_THREADS", str(ci["logical_cpus"]))
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", path,
+                            "--cpu-seconds", str(seconds)], env=env, capture_output=True,
+                           text=True, check=True)
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    times, threads = out["times"], out["threads"]
     med = float(np.median(times))
     B = spmv_bytes(A.n, A.nnz)
-    ci = cpu_info()
     return {"value": round(B / med / 1e9, 3), "unit": "GB/s", "cores": threads,
             "kind": "port",
             "GBs_16B_per_nnz": round(16.0 * A.nnz / med / 1e9, 3),
@@ -181,9 +207,9 @@ def cpu_baseline(A, seconds=10.0, min_runs=10):
             "p10_p90_ms": [round(float(np.percentile(times, q)) * 1e3, 4) for q in (10, 90)],
             "runs": len(times),
             "sample": "%d single FP64 SpMVs after one warm-up (oracle ParallelSPMV "
-                      "restatement, OpenMP, %d threads) on the same RCM-ordered 128^3 "
-                      "7-point matrix, %.1f s; value = median" % (len(times), threads,
-                                                                 sum(times)),
+                      "restatement, OpenMP, %d threads, OMP_PROC_BIND=close "
+                      "OMP_PLACES=cores) on the same RCM-ordered 128^3 7-point matrix, "
+                      "%.1f s; value = median" % (len(times), threads, sum(times)),
             "cpu_model": ci["model"], "cpu_ident": ci["ident"],
             "logical_cpus": ci["logical_cpus"], "physical_cores": ci["physical_cores"]}
 
@@ -610,7 +636,11 @@ def main():
     ap.add_argument("--c4-grid", type=int, default=160)
     ap.add_argument("--no-extra", action="store_true",
                     help="leave out the c4, c3 and c5 records of the default line")
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_child:  # the CPU baseline's own process (no GPU)
+        cpu_child(args.cpu_child, args.cpu_seconds)
+        return
     cpu_info()  # before torch or the libraries start an OpenMP runtime
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only; the multi-GPU sharding of the "
