@@ -412,12 +412,12 @@ def bench_c3(la, torch, dev, stream, args):
         big = torch.empty(1 << 27, dtype=torch.float64, device=dev)
         torch.empty_like(big).copy_(big)
         torch.cuda.synchronize()
-        return {"probe": "c3 spmv albus x4, auto (column blocks) x4, copy 1GiB", "nnz": A.nnz}
+        return {"probe": "c3 spmv albus x4, auto (row panels x column blocks) x4, copy 1GiB", "nnz": A.nnz}
     rl = np.diff(A.ai)
     mats = [la.DeviceMatrix(A, stream=stream) for _ in range(2)]
     algs = [("auto", la.SPMV_AUTO), ("albus", la.SPMV_ALBUS), ("row", la.SPMV_ROW)]
-    # the column-blocked form AUTO picks here: blocks of the matrix's columns
-    # (4 MB of FP64 x each), one ALBUS pass per block adding into y
+    # AUTO picks the row-panel x column-block kernel here: one launch, each
+    # workgroup sweeps blocks of 65,536 columns with its rows' sums in LDS
     tab = _spmv_table(la, torch, A, mats, dev, stream, algs,
                       [(la.F64, "f64"), (la.F32, "f32"), (la.BF16, "bf16")], 20)
     return {"config": "c3 power-law n=4M", "n": A.n, "nnz": A.nnz,

@@ -54,11 +54,13 @@ typedef enum {
   LA_SPMV_AUTO = 3,
   LA_SPMV_SELL = 4, /* sliced ELL copy, one lane per row (no counterpart in the
                        reference; AUTO picks it for near-uniform row lengths) */
-  LA_SPMV_COLBLOCK = 5 /* ALBUS on column blocks of the matrix, one block (a
-                          few MB of x, held by the L2 caches) after the other,
-                          each adding into y (no counterpart in the reference;
-                          AUTO picks it for ragged rows whose columns are
-                          scattered over an x larger than the L2 caches) */
+  LA_SPMV_COLBLOCK = 5 /* row panels x column blocks: each workgroup keeps a
+                          panel of rows' sums in LDS while it sweeps blocks of
+                          65,536 columns (512 KB of FP64 x, held by the L2
+                          caches) in order, and writes y once (no counterpart
+                          in the reference; AUTO picks it for ragged rows
+                          whose columns are scattered over an x larger than
+                          the L2 caches) */
 } la_spmv_alg;
 
 /* triangle_solve.hpp:147-208 OptimizedTriangularSolve modes: Barrier ->

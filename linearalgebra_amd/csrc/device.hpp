@@ -32,13 +32,18 @@ struct Mat {
   bool sell_col16 = false; // columns stored as 16-bit offsets from the row
   DevBuf sell_ptr, sell_w, sell_col, sell_src, sell_val[3];
   uint64_t sell_ver[3] = {0, 0, 0};
-  // column blocks (LA_SPMV_COLBLOCK): the entries of each block of columns
-  // as a matrix of their own (same rows), built on first use; their values
-  // are gathered on the device from A's (cb_src) when A's values change
-  int cb_state = 0;  // 0 not decided, 1 built, -1 not worth it
-  std::vector<std::unique_ptr<Mat>> cb;
-  std::vector<DevBuf> cb_src;
-  uint64_t cb_ver = ~0ull;
+  // row panels x column blocks (LA_SPMV_COLBLOCK), built on first use
+  // (cb_state: 0 not decided, 1 built, -1 not worth it): pb_np panels of rows
+  // (pb_prow), pb_nb blocks of pb_width columns; per (panel, block) tile the
+  // chunk offsets of its wavefronts (pb_chunk), the packed entries (pb_idx)
+  // and the entry of A each holds (pb_src, -1 for padding); the values
+  // follow A's through pb_src (pb_ver: the version gathered)
+  int cb_state = 0;
+  int pb_np = 0, pb_nb = 0, pb_width = 0;
+  int64_t pb_len = 0;
+  DevBuf pb_prow, pb_chunk, pb_idx, pb_src;
+  MultiValues pb_val;
+  uint64_t pb_ver = ~0ull;
   // A^T, its pattern built on first transposed product; its values are
   // gathered on the device from A's (t_src: the entry of A each entry of
   // A^T holds) whenever A's values changed (t_ver: the version gathered)

@@ -1,6 +1,6 @@
 // CSR SpMV on gfx950.
 //
-// Two kernel families (both compute y = alpha*A*x + beta*z):
+// Kernel families (all compute y = alpha*A*x + beta*z):
 //  * row kernel: G lanes of a 64-wide wavefront per row, G the power of two
 //    at or above the mean row length; coalesced row streams, DPP reduction.
 //    Replaces SerialSPMV/ParallelSPMV (sparse_mat_op/spmv.hpp:12-57).
@@ -10,6 +10,10 @@
 //    reduced from LDS, the partial head/tail rows leave one partial each and
 //    a second tiny kernel adds them in slice order (the ALBUS boundary fix-up
 //    of spmv.hpp:275-292, deterministic). Replaces SegSumSPMV and ALBUSSPMV.
+//  * sliced ELL: one lane per row of 64-row column-major slices (near-uniform
+//    row lengths).
+//  * row panels x column blocks: a workgroup per panel of rows, its sums in
+//    LDS across blocks of columns (ragged rows scattered over a large x).
 #include <algorithm>
 #include <climits>
 #include <cstdlib>
@@ -229,6 +233,139 @@ __global__ void k_spmv_seg_fix(int64_t nseg, const int *__restrict__ tail_row,
   A o = alpha * acc;
   if (beta != A(0)) o += beta * A(z[r]);
   y[r] = V(o);
+}
+
+// --------------------------------------------- row panels x column blocks
+// For products whose x gathers scatter over an x far larger than an XCD's
+// 4 MB L2. Columns are cut into blocks of at most 2^kPanelColBits columns
+// (4 MB of FP64 x) and rows into panels of at most kPanelRows rows. One
+// workgroup owns a panel: its row sums live in LDS while it sweeps the
+// blocks in column order, so every workgroup gathers from the same block at
+// about the same time (that block's lines stay in the L2s) and y is written
+// once. Inside a (panel, block) tile the entries are in CSR order and cut
+// at row boundaries into one chunk per wavefront (padded to a multiple of
+// four with kPanelPad entries), so a row of a tile belongs to one
+// wavefront. An entry is one 32-bit word: the row inside the panel above
+// kPanelColBits bits of column inside the block.
+constexpr int kPanelWaves = 8;
+constexpr int kPanelThreads = 64 * kPanelWaves;
+constexpr int kPanelRows = 8192;  // FP64 row sums: 64 KB of LDS per workgroup
+constexpr int kPanelColBits = 19;
+constexpr unsigned kPanelPad = 0xFFFFFFFFu;
+
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_rm(int v, int old) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, RM, 0xF, false);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_rm(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, RM, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, RM, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp_rm(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, RM, 0xF, false));
+}
+template <int CTRL, int RM, typename A>
+__device__ __forceinline__ void seg_scan_step(A &v, int key) {
+  const A t = dpp_rm<CTRL, RM>(v);
+  const int k = dpp_rm<CTRL, RM>(key, INT_MIN);
+  if (k == key) v += t;
+}
+// Inclusive scan of v over the lanes of a wavefront, restarted wherever the
+// key changes (equal keys occupy consecutive lanes): row_shr 1, 2, 4, 8
+// inside each 16-lane row, then row_bcast 15 (rows 1 and 3) and row_bcast 31
+// (rows 2 and 3). Every lane must be active.
+template <typename A>
+__device__ __forceinline__ A seg_scan(A v, int key) {
+  seg_scan_step<0x111, 0xF>(v, key);
+  seg_scan_step<0x112, 0xF>(v, key);
+  seg_scan_step<0x114, 0xF>(v, key);
+  seg_scan_step<0x118, 0xF>(v, key);
+  seg_scan_step<0x142, 0xA>(v, key);
+  seg_scan_step<0x143, 0xC>(v, key);
+  return v;
+}
+
+// Each lane takes four consecutive entries of its chunk (one vector load of
+// words, one of values), sums the runs of equal rows in order, and adds the
+// runs that begin and end inside its four straight into the row sums. A
+// run that touches the lane's first or last entry may continue in the
+// neighbouring lanes: the lanes' last runs are combined by a scan over
+// lanes keyed by row, and each such row is added once, by the lane where
+// it ends. Every row is therefore added to by one lane of one instruction,
+// in an order fixed by the layout: the result is deterministic.
+template <typename T, typename V, typename A>
+__global__ __launch_bounds__(kPanelThreads) void k_spmv_panel(
+    int nb, int width, const int *__restrict__ prow, const int *__restrict__ chunk,
+    const unsigned *__restrict__ pidx, const T *__restrict__ pval, const V *__restrict__ x,
+    V *y, const V *z, A alpha, A beta) {
+  using Q = typename Quad<T>::type;
+  __shared__ A ys[kPanelRows];
+  const int p = blockIdx.x;
+  const int r0 = prow[p], nr = prow[p + 1] - r0;
+  for (int i = threadIdx.x; i < nr; i += kPanelThreads) ys[i] = A(0);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int *cp = chunk + static_cast<int64_t>(p) * nb * (kPanelWaves + 1) + wave;
+  constexpr unsigned kColMask = (1u << kPanelColBits) - 1u;
+  for (int b = 0; b < nb; ++b, cp += kPanelWaves + 1) {
+    const int s = cp[0], e = cp[1];
+    const V *xb = x + static_cast<int64_t>(b) * width;
+    for (int q0 = s; q0 < e; q0 += 256) {
+      const int q = q0 + 4 * lane;
+      int rw[4];
+      A pr[4];
+      if (q < e) {  // chunks are whole quads
+        const int4v c = __builtin_nontemporal_load(reinterpret_cast<const int4v *>(pidx + q));
+        const Q v = __builtin_nontemporal_load(reinterpret_cast<const Q *>(pval + q));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const unsigned u = static_cast<unsigned>(c[j]);
+          const bool pad = u == kPanelPad;
+          rw[j] = pad ? -1 : static_cast<int>(u >> kPanelColBits);
+          const V xv = ldx(xb, pad ? 0 : static_cast<int>(u & kColMask));
+          const A t = widen<A>(v[j]) * A(xv);
+          pr[j] = pad ? A(0) : t;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { rw[j] = -1; pr[j] = A(0); }
+      }
+      // runs of equal rows inside the lane; the first is held back (it may
+      // continue an earlier lane's run), inner ones are complete
+      A run = pr[0], hs = A(0);
+      int cur = rw[0], hr = -1;
+      bool multi = false;
+#pragma unroll
+      for (int j = 1; j < 4; ++j) {
+        if (rw[j] == cur) {
+          run += pr[j];
+        } else {
+          if (!multi) { hs = run; hr = cur; multi = true; }
+          else if (cur >= 0) ys[cur] += run;
+          cur = rw[j];
+          run = pr[j];
+        }
+      }
+      // the last run: scan over the lanes whose last runs share its row
+      const A sc = seg_scan(run, cur);
+      const A sp = __shfl_up(sc, 1, 64);
+      const int kp = __shfl_up(cur, 1, 64);
+      const int first = multi ? hr : cur;
+      const int nfirst = __shfl_down(first, 1, 64);
+      if (multi && hr >= 0) ys[hr] += (lane > 0 && kp == hr) ? sp + hs : hs;
+      if (cur >= 0 && !(lane < 63 && nfirst == cur)) ys[cur] += sc;
+    }
+    __syncthreads();  // the next block's rows may fall to other wavefronts
+  }
+  for (int i = threadIdx.x; i < nr; i += kPanelThreads) {
+    A o = alpha * ys[i];
+    if (beta != A(0)) o += beta * A(z[r0 + i]);
+    y[r0 + i] = V(o);
+  }
 }
 
 // ------------------------------------------------------------------ grad
@@ -528,6 +665,17 @@ void launch_sell(Mat &M, const T *av, const V *x, V *y, const V *z, A alpha, A b
 }
 
 template <typename T, typename V, typename A>
+void launch_panel(Mat &M, const void *pv, const void *x, void *y, const void *z, double alpha,
+                  double beta, hipStream_t s) {
+  hipLaunchKernelGGL((k_spmv_panel<T, V, A>), dim3(M.pb_np), dim3(kPanelThreads), 0, s,
+                     M.pb_nb, M.pb_width, M.pb_prow.as<int>(), M.pb_chunk.as<int>(),
+                     M.pb_idx.as<unsigned>(), static_cast<const T *>(pv),
+                     static_cast<const V *>(x), static_cast<V *>(y), static_cast<const V *>(z),
+                     A(alpha), A(beta));
+  LA_HIP(hipGetLastError());
+}
+
+template <typename T, typename V, typename A>
 void dispatch_alg(Mat &M, la_spmv_alg alg, const void *av, const void *x, void *y,
                   const void *z, double alpha, double beta, hipStream_t s) {
   if (M.nnz == 0 || M.cols == 0) alg = LA_SPMV_ROW;  // nothing to gather
@@ -614,80 +762,165 @@ bool spmv_sell_rowscaled(Mat &A, la_dtype val, la_dtype vec, const void *rs, con
 }
 
 namespace {
-// out[t] = v[src[t]]
+// out[t] = v[src[t]] (src < 0: 0)
 __global__ void k_gather_vals(int64_t m, const int *__restrict__ src,
                               const double *__restrict__ v, double *out) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t < m) out[t] = v[src[t]];
+  if (t < m) {
+    const int q = src[t];
+    out[t] = q >= 0 ? v[q] : 0.0;
+  }
 }
 
 // Column blocks. A product whose x gathers land all over an x much larger
 // than an XCD's L2 (4 MB) fetches a 128-byte line for nearly every 4- or
-// 8-byte value. Cut into blocks of kColBlockBytes of FP64 x, each block's
-// gathers stay in the L2s while that block is streamed; the price is one
-// pass over y per block (beta = 1 after the first). Measured on the 4M-row
-// power-law matrix: 8 blocks 637 us against 1008 us for one ALBUS pass (16
-// blocks 869 us: the y passes outgrow the savings).
-constexpr int64_t kColBlockBytes = 4 << 20;
-constexpr int kColBlockMax = 16;
+// 8-byte value. The row-panel layout cuts the columns into blocks of
+// kColBlockCols (512 KB of FP64 x) and keeps each panel's row sums in LDS
+// across the blocks, so a block costs no pass over y. Measured on the
+// 4M-row power-law matrix (FP64): blocks of 4 MB of x 540 us, 2 MB 435-439
+// us, 1 MB 434 us, 768 KB 442 us, 512 KB 358-366 us, 384 KB 360 us, 256 KB
+// 395 us, 128 KB 536 us (below 512 KB the blocks' fixed costs dominate;
+// above it the workgroups, which drift apart by a few blocks, gather from
+// more x than an L2 holds).
+constexpr int64_t kColBlockEngage = 4 << 20;  // x bytes above which blocks pay
+constexpr int kColBlockCols = 1 << 16;
+constexpr int kColBlockMax = 1024;
+constexpr int kPanelTarget = 512;  // panels: two workgroups per CU
+
+// Panels of consecutive rows with about nnz / kPanelTarget entries each and
+// at most kPanelRows rows; per panel and block the entries in CSR order, cut
+// into kPanelWaves chunks at row boundaries (about equal entries each),
+// every chunk padded to a whole number of quads.
+void build_panels(Mat &M, int nb, int width) {
+  const int64_t target = std::max<int64_t>(1, (M.nnz + kPanelTarget - 1) / kPanelTarget);
+  std::vector<int> prow{0};
+  int64_t cnt = 0;
+  for (int r = 0; r < M.rows; ++r) {
+    if (r > prow.back() && (r - prow.back() >= kPanelRows || cnt >= target)) {
+      prow.push_back(r);
+      cnt = 0;
+    }
+    cnt += M.h_ai[r + 1] - M.h_ai[r];
+  }
+  prow.push_back(M.rows);
+  const int np = static_cast<int>(prow.size()) - 1;
+  const int nc = nb * (kPanelWaves + 1);
+  std::vector<std::vector<unsigned>> pi(np);
+  std::vector<std::vector<int>> ps(np), pc(np);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int p = 0; p < np; ++p) {
+    const int ra = prow[p], rb = prow[p + 1];
+    std::vector<std::vector<std::pair<unsigned, int>>> lists(nb);
+    for (int r = ra; r < rb; ++r)
+      for (int q = M.h_ai[r]; q < M.h_ai[r + 1]; ++q) {
+        const int c = M.h_aj[q], b = c / width;
+        lists[b].push_back({(static_cast<unsigned>(r - ra) << kPanelColBits) |
+                                static_cast<unsigned>(c - b * width),
+                            q});
+      }
+    auto &I = pi[p];
+    auto &S = ps[p];
+    auto &C = pc[p];
+    C.assign(nc, 0);
+    for (int b = 0; b < nb; ++b) {
+      const auto &L = lists[b];
+      const int64_t m = static_cast<int64_t>(L.size());
+      int64_t pos = 0;
+      for (int w = 0; w < kPanelWaves; ++w) {
+        C[b * (kPanelWaves + 1) + w] = static_cast<int>(I.size());
+        int64_t e = m;
+        if (w + 1 < kPanelWaves) {
+          e = std::max<int64_t>(pos, (w + 1) * m / kPanelWaves);
+          while (e > pos && e < m && (L[e].first >> kPanelColBits) == (L[e - 1].first >> kPanelColBits))
+            ++e;
+        }
+        for (int64_t t = pos; t < e; ++t) {
+          I.push_back(L[t].first);
+          S.push_back(L[t].second);
+        }
+        while (I.size() % 4) {
+          I.push_back(kPanelPad);
+          S.push_back(-1);
+        }
+        pos = e;
+      }
+      C[b * (kPanelWaves + 1) + kPanelWaves] = static_cast<int>(I.size());
+    }
+  }
+  std::vector<int64_t> base(np + 1, 0);
+  for (int p = 0; p < np; ++p) base[p + 1] = base[p] + static_cast<int64_t>(pi[p].size());
+  const int64_t len = base[np];
+  if (len >= INT32_MAX) throw la_error(LA_ERR_ALLOC, "panel layout exceeds int32 offsets");
+  std::vector<unsigned> gi(static_cast<size_t>(len));
+  std::vector<int> gs(static_cast<size_t>(len)), gc(static_cast<size_t>(np) * nc);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int p = 0; p < np; ++p) {
+    std::copy(pi[p].begin(), pi[p].end(), gi.begin() + base[p]);
+    std::copy(ps[p].begin(), ps[p].end(), gs.begin() + base[p]);
+    for (int k = 0; k < nc; ++k)
+      gc[static_cast<size_t>(p) * nc + k] = static_cast<int>(base[p] + pc[p][k]);
+  }
+  M.pb_np = np;
+  M.pb_nb = nb;
+  M.pb_width = width;
+  M.pb_len = len;
+  M.pb_prow.alloc(sizeof(int) * prow.size());
+  M.pb_prow.upload(prow.data(), sizeof(int) * prow.size(), M.stream);
+  M.pb_chunk.alloc(sizeof(int) * gc.size());
+  M.pb_chunk.upload(gc.data(), sizeof(int) * gc.size(), M.stream);
+  M.pb_idx.alloc(sizeof(unsigned) * std::max<int64_t>(len, 4));
+  M.pb_idx.upload(gi.data(), sizeof(unsigned) * len, M.stream);
+  M.pb_src.alloc(sizeof(int) * std::max<int64_t>(len, 4));
+  M.pb_src.upload(gs.data(), sizeof(int) * len, M.stream);
+  LA_HIP(hipStreamSynchronize(M.stream));
+}
 
 bool prepare_colblock(Mat &M) {
   if (M.cb_state) return M.cb_state > 0;
   M.cb_state = -1;
   const int64_t xbytes = static_cast<int64_t>(M.cols) * 8;
-  const int nb = static_cast<int>((xbytes + kColBlockBytes - 1) / kColBlockBytes);
-  if (nb < 2 || nb > kColBlockMax || M.nnz == 0) return false;
-  const int64_t width = (M.cols + nb - 1) / nb;
-  // scattered: most columns farther from their row than a quarter block
-  // (a banded matrix keeps its gathers near the row: one pass is better)
+  if (xbytes <= kColBlockEngage || M.nnz == 0) return false;
+  const int nb = static_cast<int>((M.cols + kColBlockCols - 1) / kColBlockCols);
+  if (nb > kColBlockMax) return false;
+  const int width = static_cast<int>((M.cols + nb - 1) / nb);
+  // scattered: most columns farther from their row than a quarter of the
+  // 4 MB reach (a banded matrix keeps its gathers near the row: one pass
+  // is better)
+  const int64_t reach = kColBlockEngage / 8;
   int64_t far = 0;
   for (int r = 0; r < M.rows; ++r)
     for (int q = M.h_ai[r]; q < M.h_ai[r + 1]; ++q)
-      far += std::llabs(static_cast<int64_t>(M.h_aj[q]) - r) > width / 4;
+      far += std::llabs(static_cast<int64_t>(M.h_aj[q]) - r) > reach / 4;
   if (far * 2 < M.nnz) return false;
-  std::vector<std::vector<int>> bai(nb, std::vector<int>(static_cast<size_t>(M.rows) + 1, 0)),
-      baj(nb), bsrc(nb);
-  for (int r = 0; r < M.rows; ++r) {
-    for (int q = M.h_ai[r]; q < M.h_ai[r + 1]; ++q) {
-      const int b = static_cast<int>(M.h_aj[q] / width);
-      baj[b].push_back(M.h_aj[q]);
-      bsrc[b].push_back(q);
-    }
-    for (int b = 0; b < nb; ++b) bai[b][r + 1] = static_cast<int>(baj[b].size());
-  }
-  for (int b = 0; b < nb; ++b) {
-    std::vector<double> zeros(baj[b].size(), 0.0);
-    M.cb.push_back(make_mat(M.rows, M.cols, 0, bai[b].data(), baj[b].data(), zeros.data(),
-                            M.stream));
-    DevBuf d;
-    d.alloc(sizeof(int) * std::max<size_t>(bsrc[b].size(), 1));
-    d.upload(bsrc[b].data(), sizeof(int) * bsrc[b].size(), M.stream);
-    M.cb_src.push_back(std::move(d));
-  }
-  LA_HIP(hipStreamSynchronize(M.stream));
-  M.cb_ver = ~0ull;
+  build_panels(M, nb, width);
+  M.pb_ver = ~0ull;
   M.cb_state = 1;
   return true;
 }
 
-// the blocks' values follow A's (every stored precision)
+// the panel layout's values follow A's (every stored precision)
 void colblock_values(Mat &M, la_dtype val, hipStream_t s) {
-  if (M.cb_ver != M.vals.version) {
-    for (size_t b = 0; b < M.cb.size(); ++b) {
-      Mat &B = *M.cb[b];
-      if (B.nnz) {
-        double *bv = B.vals.v[LA_F64].as<double>();
-        hipLaunchKernelGGL(k_gather_vals, dim3(static_cast<unsigned>((B.nnz + 255) / 256)),
-                           dim3(256), 0, s, B.nnz, M.cb_src[b].as<int>(),
-                           static_cast<const double *>(M.vals.get(LA_F64)), bv);
-        LA_HIP(hipGetLastError());
-        B.vals.set_from_device_f64(bv, B.nnz, s);
-      }
+  if (M.pb_ver != M.vals.version) {
+    const unsigned g = static_cast<unsigned>((M.pb_len + 255) / 256);
+    const double *av = static_cast<const double *>(M.vals.get(LA_F64));
+    if (M.pb_val.has[LA_F64]) {
+      double *pv = M.pb_val.v[LA_F64].as<double>();
+      hipLaunchKernelGGL(k_gather_vals, dim3(g), dim3(256), 0, s, M.pb_len,
+                         M.pb_src.as<int>(), av, pv);
+      LA_HIP(hipGetLastError());
+      M.pb_val.set_from_device_f64(pv, M.pb_len, s);
+    } else {
+      DevBuf t;
+      t.alloc(sizeof(double) * std::max<int64_t>(M.pb_len, 1));
+      hipLaunchKernelGGL(k_gather_vals, dim3(g), dim3(256), 0, s, M.pb_len,
+                         M.pb_src.as<int>(), av, t.as<double>());
+      LA_HIP(hipGetLastError());
+      M.pb_val.set_from_device_f64(t.as<double>(), M.pb_len, s);
+      LA_HIP(hipStreamSynchronize(s));
     }
-    M.cb_ver = M.vals.version;
+    M.pb_ver = M.vals.version;
   }
-  for (auto &B : M.cb)
-    if (!B->vals.has[val]) B->vals.add(val, s);
+  if (!M.pb_val.has[val]) M.pb_val.add(val, s);
 }
 }  // namespace
 
@@ -729,9 +962,20 @@ void spmv(Mat &A, la_spmv_alg alg, la_dtype val, la_dtype vec, la_dtype acc, dou
       spmv_resolve(A, alg) == LA_SPMV_COLBLOCK) {
     // y = alpha A_0 x + beta z, then y += alpha A_b x for the other blocks
     colblock_values(A, val, s);
-    for (size_t b = 0; b < A.cb.size(); ++b)
-      spmv(*A.cb[b], LA_SPMV_ALBUS, val, vec, acc, alpha, x, b == 0 ? beta : 1.0,
-           b == 0 ? z : y, y, s);
+    const void *pv = A.pb_val.get(val);
+    if (acc == LA_F64) {
+      LA_CHECK_ARG(val == LA_F64 && vec == LA_F64,
+                   "FP64 accumulation needs FP64 values and vectors");
+      launch_panel<double, double, double>(A, pv, x, y, z, alpha, beta, s);
+    } else if (vec == LA_F64) {
+      if (val == LA_F64) launch_panel<double, double, float>(A, pv, x, y, z, alpha, beta, s);
+      else if (val == LA_F32) launch_panel<float, double, float>(A, pv, x, y, z, alpha, beta, s);
+      else launch_panel<bf16, double, float>(A, pv, x, y, z, alpha, beta, s);
+    } else {
+      if (val == LA_F64) launch_panel<double, float, float>(A, pv, x, y, z, alpha, beta, s);
+      else if (val == LA_F32) launch_panel<float, float, float>(A, pv, x, y, z, alpha, beta, s);
+      else launch_panel<bf16, float, float>(A, pv, x, y, z, alpha, beta, s);
+    }
     return;
   }
   if (alg == LA_SPMV_COLBLOCK) alg = LA_SPMV_ALBUS;

@@ -278,11 +278,10 @@ def test_documented_spmv_deviations(dev):
 
 def test_colblock_scattered_matrix(dev):
     """A ragged matrix with columns scattered over an x larger than the L2
-    caches: AUTO runs the column-blocked form (ALBUS per column block, each
-    adding into y); alpha/beta (beta = 1 makes every pass accumulate), the
-    FP32/BF16 value
-    copies and value updates all follow, against the oracle (FP64 1e-12
-    element-wise)."""
+    caches: AUTO runs the row-panel x column-block kernel (each panel's row
+    sums held across the blocks); alpha/beta, the FP32/BF16 value copies and
+    value updates all follow, against the oracle (FP64 1e-12 element-wise),
+    and repeated products are bitwise equal."""
     import torch
     A = _o(la.gen.powerlaw(700_000, 8.0, 1.0, 3000, seed=6))
     dA = la.DeviceMatrix(as_lib_csr(A))
@@ -308,6 +307,60 @@ def test_colblock_scattered_matrix(dev):
     dA.spmv(xt, yt, alg=la.SPMV_AUTO)
     A2 = orc.CSR(A.n, A.ai, A.aj, av)
     assert np.all(np.abs(yt.cpu().numpy() - orc.spmv(A2, x)) <= 1e-12 * _absbound(A2, x) + 1e-300)
+    for _ in range(3):
+        y2 = torch.zeros_like(yt)
+        dA.spmv(xt, y2, alg=la.SPMV_AUTO)
+        assert torch.equal(y2, yt)
+
+
+def _scattered(n, ncols, lens, seed):
+    """rows of the given lengths, distinct uniform random columns in CSR order"""
+    rng = np.random.default_rng(seed)
+    ai = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    aj = np.empty(ai[-1], np.int64)
+    for r in np.flatnonzero(lens > 1):
+        aj[ai[r]:ai[r + 1]] = np.sort(rng.choice(ncols, lens[r], replace=False))
+    one = np.flatnonzero(lens == 1)
+    aj[ai[one]] = rng.integers(0, ncols, len(one))
+    return orc.CSR(n, ai, aj, rng.standard_normal(ai[-1]), ncols=ncols)
+
+
+@pytest.mark.parametrize("case", ["short_rows_full_panels", "rect_long_rows"])
+def test_colblock_panel_shapes(dev, case):
+    """Edge shapes of the row-panel x column-block layout: one entry per row
+    over 4.3M rows with every 7th row empty (panels capped at 8192 rows, so
+    more panels than workgroups fit at once), and a 60000 x 3.1M matrix whose
+    rows reach 6000 entries (a row spans many lanes, several 256-entry steps
+    and every block). FP64 against the oracle (1e-12 element-wise), FP32
+    values against the oracle on rounded values, repeats bitwise equal."""
+    import torch
+    if case == "short_rows_full_panels":
+        n = ncols = 4_300_000
+        lens = np.ones(n, np.int64)
+        lens[::7] = 0
+    else:
+        n, ncols = 60_000, 3_100_000
+        lens = np.random.default_rng(2).integers(0, 40, n)
+        lens[::997] = 6000
+    A = _scattered(n, ncols, lens, 11)
+    dA = la.DeviceMatrix(as_lib_csr(A))
+    assert dA.resolve(la.SPMV_AUTO) == la.SPMV_COLBLOCK
+    x = la.gen.uniform(ncols, 4)
+    y0 = la.gen.uniform(n, 5)
+    xt = torch.tensor(x, device=dev)
+    yt = torch.tensor(y0, device=dev)
+    dA.spmv(xt, yt, alpha=1.5, beta=-2.0, alg=la.SPMV_AUTO)
+    ref = orc.spmv(A, x, y0, alpha=1.5, beta=-2.0)
+    scale = 1.5 * _absbound(A, x) + 2.0 * np.abs(y0)
+    got = yt.cpu().numpy()
+    assert np.all(np.abs(got - ref) <= 1e-12 * scale + 1e-300)
+    y2 = torch.tensor(y0, device=dev)
+    dA.spmv(xt, y2, alpha=1.5, beta=-2.0, alg=la.SPMV_AUTO)
+    assert torch.equal(y2, yt)
+    yf = torch.zeros(n, dtype=torch.float32, device=dev)
+    dA.spmv(xt.float(), yf, alg=la.SPMV_AUTO, val=la.F32, acc=la.F32)
+    ref_r = orc.spmv(_round_values(A, la.F32), x)
+    assert np.all(np.abs(yf.double().cpu().numpy() - ref_r) <= ACC32_BOUND * _absbound(A, x))
 
 
 def test_value_grad(dev):
