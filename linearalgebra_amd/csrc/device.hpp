@@ -124,6 +124,7 @@ struct TrsvPlan {
   // level-ordered values)
   bool tiled = false;
   int t_nparts = 0, t_ngroups = 0, t_mmax = 0, t_hmax = 0, t_maxe = 0, t_tau = 0;
+  int t_rlanes = 1;  // lanes per row (kTileRowLanes for rows of more than 8 entries)
   double t_remote = 0;
   DevBuf t_gptr, t_prow, t_pcode, t_rfptr, t_rfcol, t_rfslot;
   DevBuf t_lsrc, t_dpos;  // slot entry -> level-ordered entry; slot -> level position (-1: padding)

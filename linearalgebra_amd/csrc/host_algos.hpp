@@ -48,23 +48,28 @@ bool ick_numeric(int n, int base, const int *ai, const int *aj, const double *av
 
 // Tiled schedule of the sync-free triangular solve (tiling.cpp): rows cut
 // into tiles of a grid the pattern is recognised as, each tile's rows in
-// (level, index) order in groups of at most `lanes` rows of one level.
-constexpr int kTileMaxEntries = 8;  // longest row the tiled kernel takes
+// (level, index) order in groups of at most `lanes` / rlanes rows of one
+// level. Rows of up to kTileMaxEntries entries take one lane; longer rows,
+// up to kTileRowLanes * kTileMaxEntries, take kTileRowLanes lanes.
+constexpr int kTileMaxEntries = 8;  // entries per lane
+constexpr int kTileRowLanes = 4;    // lanes of a long row
 constexpr int kTileWindow = 16;     // groups a tile keeps its results for
 struct TileLayout {
   int nparts = 0, ngroups = 0;
+  int rlanes = 1;  // lanes per row (a row's entries are split in order)
   int m_max = 0;   // rows of the largest tile
   int h_max = 0;   // most values one tile reads from other tiles
-  int maxe = 0;    // entries per row slot (4 or kTileMaxEntries)
+  int maxe = 0;    // entries per lane slot (4 or kTileMaxEntries)
   int tau = 0;     // tile edge in grid steps
   double remote_frac = 0;  // entries read from another tile
   std::vector<int> part_gptr;  // tile p: groups [part_gptr[p], part_gptr[p+1])
   std::vector<int> grp_row;    // group g: row positions [grp_row[g], grp_row[g+1])
   std::vector<int> tperm;      // row position -> row
-  // per row position, maxe entries: >= 0 an LDS slot of the tile's own
-  // earlier rows ((group % kTileWindow) * lanes + row in group; the slot
-  // kTileWindow * lanes reads 0 and pads short rows), < 0 the halo slot
-  // -(code+1); tsrc: the entry's index in the row-order triangle (-1: pad)
+  // per row position, rlanes slots of maxe entries: >= 0 an LDS slot of
+  // the tile's own earlier rows ((group % kTileWindow) * lanes + rlanes *
+  // row in group; the slot kTileWindow * lanes reads 0 and pads short
+  // rows), < 0 the halo slot -(code+1); tsrc: the entry's index in the
+  // row-order triangle (-1: pad)
   std::vector<int> tcode, tsrc;
   // group g's first reads from other tiles: [rf_ptr[g], rf_ptr[g+1]) of
   // (rf_col: the row, rf_slot: its halo slot)

@@ -269,14 +269,15 @@ struct TileStage {
   T dinv;
 };
 
-template <int E, typename T, typename V, typename A, bool DIAG>
+template <int E, int RL, typename T, typename V, typename A, bool DIAG>
 __global__ __launch_bounds__(kTileLanes * (1 + kTileHalo)) void k_trsv_tiled(
     int nparts, const int *__restrict__ gptr, const int *__restrict__ prow,
     const uint16_t *__restrict__ pcode, const T *__restrict__ pval, const T *__restrict__ pdinv,
     const int *__restrict__ rfptr, const int *__restrict__ rfcol, const int *__restrict__ rfslot,
     const V *__restrict__ b, V *x, unsigned *ctl, int *err) {
-  // pipeline depth: rows of 8 entries hold twice the registers
-  constexpr int D = E <= 4 ? kTileDepth : kTileDepth * 3 / 4, H = kTileHalo;
+  // pipeline depth: slots of 8 entries hold twice the registers
+  constexpr int D = E <= 4 ? kTileDepth : kTileDepth * 3 / 4;
+  constexpr int H = kTileHalo;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_prog[H];
   __shared__ int s_part, s_late;
@@ -357,6 +358,9 @@ __global__ __launch_bounds__(kTileLanes * (1 + kTileHalo)) void k_trsv_tiled(
           A acc = A(tile_val(st[k].val[0])) * A(xv[0]);
 #pragma unroll
           for (int e = 1; e < E; ++e) acc += A(tile_val(st[k].val[e])) * A(xv[e]);
+          // a row on RL lanes: every lane gets the same sum (the first
+          // lane's result is the one read and published)
+          acc = group_sum<RL>(acc);
           A v = A(bv[k]) - acc;
           if constexpr (DIAG) v = v * A(tile_val(st[k].dinv));
           const V out = V(v);
@@ -492,9 +496,9 @@ inline size_t tile_lds_bytes(int h_max, size_t vsize) {
   return (static_cast<size_t>(kTileXl) + 1 + h_max) * vsize;
 }
 
-template <int E, typename T, typename V, typename A, bool DIAG>
+template <int E, int RL, typename T, typename V, typename A, bool DIAG>
 void tiled_launch(TrsvPlan &P, const V *b, V *x, hipStream_t s) {
-  auto kern = k_trsv_tiled<E, T, V, A, DIAG>;
+  auto kern = k_trsv_tiled<E, RL, T, V, A, DIAG>;
   const size_t lds = tile_lds_bytes(P.t_hmax, sizeof(V));
   static size_t lds_set = 0;  // per instantiation: the largest size allowed so far
   if (lds > lds_set) {
@@ -518,8 +522,9 @@ template <typename T, typename V, typename A, bool DIAG>
 void launch_syncfree(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x, hipStream_t s) {
   const int64_t n = P.n;
   if (P.tiled) {
-    if (P.t_maxe == 4) tiled_launch<4, T, V, A, DIAG>(P, b, x, s);
-    else tiled_launch<kTileMaxEntries, T, V, A, DIAG>(P, b, x, s);
+    if (P.t_maxe == 4) tiled_launch<4, 1, T, V, A, DIAG>(P, b, x, s);
+    else if (P.t_rlanes == 1) tiled_launch<kTileMaxEntries, 1, T, V, A, DIAG>(P, b, x, s);
+    else tiled_launch<kTileMaxEntries, kTileRowLanes, T, V, A, DIAG>(P, b, x, s);
     LA_HIP(hipGetLastError());
     return;
   }
@@ -776,28 +781,34 @@ void build_tiled(TrsvPlan &P, const std::vector<int> &iperm, const std::vector<i
   if (!build_tile_layout(n, P.h_ai.data(), P.h_aj.data(), level.data(), kTileLanes, ncu, n,
                          max_halo, L, P.t_why))
     return;
-  const int E = L.maxe;
+  const int E = L.maxe, RL = L.rlanes;
   const int G = L.ngroups;
   const int64_t slots = static_cast<int64_t>(G) * kTileLanes, m = slots * E;
   std::vector<int> prow(static_cast<size_t>(slots), -1), dpos(static_cast<size_t>(slots), -1);
   std::vector<int> lsrc(static_cast<size_t>(m), -1);
   std::vector<uint16_t> pcode(static_cast<size_t>(m), static_cast<uint16_t>(kTileXl));
   std::vector<double> tv(static_cast<size_t>(m), 0.0), td(static_cast<size_t>(slots), 0.0);
+  // a row of RL lanes: its first lane carries the row id, 1/d and the
+  // level position; the others only entries (no id: they publish nothing)
   for (int g = 0; g < G; ++g)
-    for (int kk = L.grp_row[g]; kk < L.grp_row[g + 1]; ++kk) {
-      const int64_t sl = static_cast<int64_t>(g) * kTileLanes + (kk - L.grp_row[g]);
-      const int r = L.tperm[kk];
-      prow[sl] = r;
-      dpos[sl] = lpos[r];
-      if (!P.unit) td[sl] = 1.0 / P.h_diag[r];
-      for (int e = 0; e < E; ++e) {
-        const size_t te = static_cast<size_t>(kk) * E + e, pe = static_cast<size_t>(sl) * E + e;
-        const int c = L.tcode[te], q = L.tsrc[te];
-        pcode[pe] = static_cast<uint16_t>(c >= 0 ? c : kTileXl + 1 + (-c - 1));
-        lsrc[pe] = q >= 0 ? lrp[lpos[r]] + (q - P.h_ai[r]) : -1;
-        tv[pe] = q >= 0 ? P.h_av[q] : 0.0;
+    for (int kk = L.grp_row[g]; kk < L.grp_row[g + 1]; ++kk)
+      for (int w = 0; w < RL; ++w) {
+        const int64_t sl = static_cast<int64_t>(g) * kTileLanes + RL * (kk - L.grp_row[g]) + w;
+        const int r = L.tperm[kk];
+        if (w == 0) {
+          prow[sl] = r;
+          dpos[sl] = lpos[r];
+          if (!P.unit) td[sl] = 1.0 / P.h_diag[r];
+        }
+        for (int e = 0; e < E; ++e) {
+          const size_t te = (static_cast<size_t>(kk) * RL + w) * E + e;
+          const size_t pe = static_cast<size_t>(sl) * E + e;
+          const int c = L.tcode[te], q = L.tsrc[te];
+          pcode[pe] = static_cast<uint16_t>(c >= 0 ? c : kTileXl + 1 + (-c - 1));
+          lsrc[pe] = q >= 0 ? lrp[lpos[r]] + (q - P.h_ai[r]) : -1;
+          tv[pe] = q >= 0 ? P.h_av[q] : 0.0;
+        }
       }
-    }
   std::vector<int> rfslot(L.rf_slot.size());
   for (size_t t = 0; t < rfslot.size(); ++t) rfslot[t] = kTileXl + 1 + L.rf_slot[t];
   auto up = [&](DevBuf &d, const void *h, size_t bytes) {
@@ -819,6 +830,7 @@ void build_tiled(TrsvPlan &P, const std::vector<int> &iperm, const std::vector<i
   P.t_mmax = L.m_max;
   P.t_hmax = L.h_max;
   P.t_maxe = E;
+  P.t_rlanes = RL;
   P.t_tau = L.tau;
   P.t_remote = L.remote_frac;
   P.t_wgs = static_cast<unsigned>(std::min(L.nparts, ncu));

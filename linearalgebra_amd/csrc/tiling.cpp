@@ -18,6 +18,23 @@
 // independent axes, rows longer than the kernel takes, too many values
 // crossing tiles, tiles too large for LDS) the layout is refused and the
 // solve keeps the level-chunk schedule.
+//
+// Rows longer than one lane takes (a 27-point stencil's ILU(1) factor: up
+// to 31 entries) come from wider stencils, whose BFS metric is not the
+// grid's. For those, a natural-order box grid (row = i + nx (j + ny k)) is
+// recognised from the row indices instead, and a tile is a set of
+// kBoxTileU x kBoxTileK lines along i: the factor's levels grow by one
+// along i and by more along j and k, so a tile's rows of one level are at
+// most one per line. A row (i, j, k) of the lower triangle reads lines
+// (j - 1, k), (j - 2, k) and (j - 1 .. j + 2, k - 1), so in the skewed
+// coordinate u = j + 2k every line it reads has u' <= u and k' <= k:
+// tiles cut in (u, k) read only from tiles at lower (u, k) (higher for an
+// upper triangle), and handing them out by kBoxTileU * (u block) +
+// kBoxTileK * (k block) (reversed for an upper triangle) gives every tile
+// a later ticket than the tiles it reads from, so the solve drains
+// whatever number of tiles is resident. (Tiles cut in (j, k) would read
+// their neighbour at higher j in the plane below, which reads back from
+// them.)
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -50,6 +67,67 @@ void bfs(int n, const std::vector<int> &ptr, const std::vector<int> &adj, int s,
   }
 }
 
+// Box grids: every entry within kBoxReach steps of its row along each axis
+constexpr int kBoxReach = 2;
+constexpr int kBoxTileU = 8, kBoxTileK = 2;  // kBoxTileU * kBoxTileK = 64 / kTileRowLanes
+
+bool box_ok(int n, const int *ai, const int *aj, int64_t nx, int64_t ny) {
+  const int64_t P = nx * ny;
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int r = 0; r < n; ++r) {
+    const int64_t ri = r % nx, rj = (r / nx) % ny, rk = r / P;
+    for (int q = ai[r]; q < ai[r + 1]; ++q) {
+      const int64_t c = aj[q];
+      const int64_t ci = c % nx, cj = (c / nx) % ny, ck = c / P;
+      bad |= std::llabs(ci - ri) > kBoxReach || std::llabs(cj - rj) > kBoxReach ||
+             std::llabs(ck - rk) > kBoxReach;
+    }
+  }
+  return !bad;
+}
+
+// smallest |row - column| above `above` (INT64_MAX if none)
+int64_t min_offset_above(int n, const int *ai, const int *aj, int64_t above) {
+  int64_t m = INT64_MAX;
+#pragma omp parallel for schedule(static) reduction(min : m)
+  for (int r = 0; r < n; ++r)
+    for (int q = ai[r]; q < ai[r + 1]; ++q) {
+      const int64_t d = std::llabs(static_cast<int64_t>(aj[q]) - r);
+      if (d > above && d < m) m = d;
+    }
+  return m;
+}
+
+// nx, ny, nz of a natural-order box grid holding the pattern: the smallest
+// offset past the reach along i is a line minus at most kBoxReach, the
+// smallest past the reach along j a plane give or take kBoxReach lines and
+// steps; each candidate is checked against every entry.
+bool detect_box_grid(int n, const int *ai, const int *aj, int &nx, int &ny, int &nz) {
+  constexpr int R = kBoxReach;
+  const int64_t d1 = min_offset_above(n, ai, aj, 2 * R);
+  if (d1 == INT64_MAX) return false;
+  for (int64_t cx = d1; cx <= d1 + R; ++cx) {
+    if (cx < 2 * R + 1 || n % cx) continue;
+    const int64_t d2 = min_offset_above(n, ai, aj, R * cx + R);
+    if (d2 == INT64_MAX) continue;
+    for (int a = -R; a <= R; ++a)
+      for (int b = -R; b <= R; ++b) {
+        const int64_t P = d2 + a * cx + b;
+        if (P <= 0 || P % cx || n % P) continue;
+        const int64_t cy = P / cx, cz = n / P;
+        if (cy < 2 * R + 1 || cz < 2 * R + 1) continue;
+        if (box_ok(n, ai, aj, cx, cy)) {
+          nx = static_cast<int>(cx);
+          ny = static_cast<int>(cy);
+          nz = static_cast<int>(cz);
+          return true;
+        }
+      }
+  }
+  return false;
+}
+
 double correlation(const std::vector<int> &a, const std::vector<int> &b) {
   const size_t n = a.size();
   double ma = 0, mb = 0;
@@ -70,22 +148,10 @@ double correlation(const std::vector<int> &a, const std::vector<int> &b) {
   return sab / std::sqrt(saa * sbb);
 }
 
-}  // namespace
-
-bool build_tile_layout(int n, const int *ai, const int *aj, const int *level, int lanes,
-                       int max_parts, int max_rows, int max_halo, TileLayout &out,
-                       std::string &why) {
-  out = TileLayout();
-  if (n < 4 * lanes) {
-    why = "too few rows";
-    return false;
-  }
-  int maxe = 0;
-  for (int i = 0; i < n; ++i) maxe = std::max(maxe, ai[i + 1] - ai[i]);
-  if (maxe > kTileMaxEntries) {
-    why = "rows longer than the tiled kernel takes";
-    return false;
-  }
+// Tiles from two axes recovered by BFS (nearest-neighbour stencils, any
+// row order): part[i] per row, the tile edge and the tile count.
+bool bfs_tiles(int n, const int *ai, const int *aj, int max_parts, int max_rows,
+               std::vector<int> &part, int &tau, int &nparts, std::string &why) {
   const int64_t nnz = ai[n];
   // symmetric adjacency of the triangle (T + T^T)
   std::vector<int> ptr(static_cast<size_t>(n) + 1, 0), adj(static_cast<size_t>(2 * nnz));
@@ -150,8 +216,9 @@ bool build_tile_layout(int n, const int *ai, const int *aj, const int *level, in
   const int vmax = *std::max_element(v.begin(), v.end());
   // the smallest square tile whose count fits the resident workgroups and
   // whose rows fit LDS
-  std::vector<int> part(static_cast<size_t>(n));
-  int tau = 0, nparts = 0;
+  part.assign(static_cast<size_t>(n), 0);
+  tau = 0;
+  nparts = 0;
   for (int t = 2; t <= 4096; t *= 2) {
     const int nu = umax / t + 1, nv = vmax / t + 1;
     if (static_cast<int64_t>(nu) * nv > 4 * static_cast<int64_t>(max_parts)) continue;
@@ -176,6 +243,60 @@ bool build_tile_layout(int n, const int *ai, const int *aj, const int *level, in
     why = "no tile size fits";
     return false;
   }
+  return true;
+}
+
+}  // namespace
+
+bool build_tile_layout(int n, const int *ai, const int *aj, const int *level, int lanes,
+                       int max_parts, int max_rows, int max_halo, TileLayout &out,
+                       std::string &why) {
+  out = TileLayout();
+  if (n < 4 * lanes) {
+    why = "too few rows";
+    return false;
+  }
+  int maxe = 0;
+  for (int i = 0; i < n; ++i) maxe = std::max(maxe, ai[i + 1] - ai[i]);
+  const int64_t nnz = ai[n];
+  std::vector<int> part, box_key;
+  int tau = 0, nparts = 0, rl = 1;
+  bool box = false;
+  if (maxe <= kTileMaxEntries) {
+    if (!bfs_tiles(n, ai, aj, max_parts, max_rows, part, tau, nparts, why)) return false;
+  } else if (maxe <= kTileRowLanes * kTileMaxEntries) {
+    int nx = 0, ny = 0, nz = 0;
+    if (!detect_box_grid(n, ai, aj, nx, ny, nz)) {
+      why = "rows longer than one lane takes, and no natural-order box grid";
+      return false;
+    }
+    static_assert(kBoxTileU * kBoxTileK * kTileRowLanes == 64, "one group per level");
+    // (u, k) blocks, u = j + 2k; ids of the blocks that hold rows
+    const int nu = (ny - 1 + 2 * (nz - 1)) / kBoxTileU + 1;
+    const int nkb = (nz - 1) / kBoxTileK + 1;
+    const int64_t P = static_cast<int64_t>(nx) * ny;
+    std::vector<int> raw(static_cast<size_t>(n)), id(static_cast<size_t>(nu) * nkb, -1);
+    for (int r = 0; r < n; ++r) {
+      const int j = static_cast<int>((r / nx) % ny), k = static_cast<int>(r / P);
+      raw[r] = (k / kBoxTileK) * nu + (j + 2 * k) / kBoxTileU;
+    }
+    nparts = 0;
+    for (int r = 0; r < n; ++r)
+      if (id[raw[r]] < 0) id[raw[r]] = nparts++;
+    part.resize(static_cast<size_t>(n));
+    box_key.assign(static_cast<size_t>(nparts), 0);
+    for (int r = 0; r < n; ++r) {
+      part[r] = id[raw[r]];
+      box_key[part[r]] = kBoxTileU * (raw[r] % nu) + kBoxTileK * (raw[r] / nu);
+    }
+    tau = kBoxTileU;
+    rl = kTileRowLanes;
+    box = true;
+  } else {
+    why = "rows longer than the tiled kernel takes";
+    return false;
+  }
+  const int rows_per_group = lanes / rl;
   // rows of each tile by (level, index); tiles by their first level
   std::vector<int> order(static_cast<size_t>(n));
   for (int i = 0; i < n; ++i) order[i] = i;
@@ -189,9 +310,23 @@ bool build_tile_layout(int n, const int *ai, const int *aj, const int *level, in
   for (int p = 0; p < nparts; ++p) pstart[p + 1] += pstart[p];
   std::vector<int> porder(static_cast<size_t>(nparts));
   for (int p = 0; p < nparts; ++p) porder[p] = p;
-  std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) {
-    return level[order[pstart[a]]] < level[order[pstart[b]]];
-  });
+  if (box) {
+    // by kBoxTileU * (u block) + kBoxTileK * (k block), descending for an
+    // upper triangle
+    bool upper = false;
+    for (int i = 0; i < n; ++i)
+      if (ai[i] < ai[i + 1]) {
+        upper = aj[ai[i]] > i;
+        break;
+      }
+    std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) {
+      return upper ? box_key[a] > box_key[b] : box_key[a] < box_key[b];
+    });
+  } else {
+    std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) {
+      return level[order[pstart[a]]] < level[order[pstart[b]]];
+    });
+  }
   // row positions: tiles in ticket order, rows by (level, index) inside
   std::vector<int> prow0(static_cast<size_t>(nparts));
   out.tperm.resize(static_cast<size_t>(n));
@@ -203,9 +338,9 @@ bool build_tile_layout(int n, const int *ai, const int *aj, const int *level, in
     prow0[p] = k;
     for (int t = pstart[p]; t < pstart[p + 1]; ++t) {
       const int r = order[t];
-      // a new group at a new level or after `lanes` rows
+      // a new group at a new level or after rows_per_group rows
       const int gs = out.grp_row.back();
-      if (k > prow0[p] && (level[r] != level[out.tperm[k - 1]] || k - gs >= lanes))
+      if (k > prow0[p] && (level[r] != level[out.tperm[k - 1]] || k - gs >= rows_per_group))
         out.grp_row.push_back(k);
       else if (k == prow0[p] && k > 0)
         out.grp_row.push_back(k);
@@ -219,13 +354,15 @@ bool build_tile_layout(int n, const int *ai, const int *aj, const int *level, in
   // ends where the next one starts
   out.nparts = nparts;
   out.ngroups = static_cast<int>(out.grp_row.size()) - 1;
-  // entry codes in tiled order, maxe per row: a tile's own earlier rows
-  // live in a window of the last kTileWindow groups, row `lane` of the
-  // tile's group i in slot (i % kTileWindow) * lanes + lane; a value of
-  // another tile in halo slot h is -(h+1); padding reads the zero slot
-  // kTileWindow * lanes. Also the source entry of each (row entries keep
-  // their stored order) and the halo fetch lists.
-  out.maxe = maxe <= 4 ? 4 : kTileMaxEntries;
+  // entry codes in tiled order, rl slots of maxe entries per row: a tile's
+  // own earlier rows live in a window of the last kTileWindow groups, row
+  // w of the tile's group i in slot (i % kTileWindow) * lanes + rl * w
+  // (the first of its lanes); a value of another tile in halo slot h is
+  // -(h+1); padding reads the zero slot kTileWindow * lanes. Also the
+  // source entry of each (row entries keep their stored order, the first
+  // maxe on the row's first lane) and the halo fetch lists.
+  out.rlanes = rl;
+  out.maxe = rl == 1 && maxe <= 4 ? 4 : kTileMaxEntries;
   const int E = out.maxe;
   const int zero_slot = kTileWindow * lanes;
   std::vector<int> slot_of(static_cast<size_t>(n)), grp_of(static_cast<size_t>(n));
@@ -234,10 +371,10 @@ bool build_tile_layout(int n, const int *ai, const int *aj, const int *level, in
       for (int kk = out.grp_row[g]; kk < out.grp_row[g + 1]; ++kk) {
         grp_of[out.tperm[kk]] = g - out.part_gptr[pi];
         slot_of[out.tperm[kk]] =
-            ((g - out.part_gptr[pi]) % kTileWindow) * lanes + (kk - out.grp_row[g]);
+            ((g - out.part_gptr[pi]) % kTileWindow) * lanes + rl * (kk - out.grp_row[g]);
       }
-  out.tcode.assign(static_cast<size_t>(n) * E, zero_slot);
-  out.tsrc.assign(static_cast<size_t>(n) * E, -1);
+  out.tcode.assign(static_cast<size_t>(n) * rl * E, zero_slot);
+  out.tsrc.assign(static_cast<size_t>(n) * rl * E, -1);
   out.rf_ptr.assign(1, 0);
   std::vector<int> halo_of(static_cast<size_t>(n), -1), halo_used;
   int64_t remote = 0;
@@ -268,8 +405,9 @@ bool build_tile_layout(int n, const int *ai, const int *aj, const int *level, in
             }
             code = -(halo_of[c] + 1);
           }
-          out.tcode[static_cast<size_t>(kk) * E + e] = code;
-          out.tsrc[static_cast<size_t>(kk) * E + e] = q;
+          const size_t te = (static_cast<size_t>(kk) * rl + e / E) * E + e % E;
+          out.tcode[te] = code;
+          out.tsrc[te] = q;
         }
       }
       out.rf_ptr.push_back(static_cast<int>(out.rf_col.size()));
@@ -282,7 +420,9 @@ bool build_tile_layout(int n, const int *ai, const int *aj, const int *level, in
     why = "tile faces too large for LDS";
     return false;
   }
-  if (out.remote_frac > 0.35) {
+  // (box tiles are narrow: most of their reads cross tiles, but each
+  // value is fetched once and the halo is bounded above)
+  if (!box && out.remote_frac > 0.35) {
     why = "too many values cross tiles";
     return false;
   }

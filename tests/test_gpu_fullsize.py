@@ -130,6 +130,8 @@ def test_c4_spmv_and_ilu1_apply_full(dev):
     del d
     F = la.iluk(A, 1)
     P = la.Preconditioner(F, "ilu")
+    print("ILU(1) tiles", P.tiles(), flush=True)
+    assert min(P.tiles()) > 0  # the box-grid tiled schedule
     r = la.gen.uniform(A.n, 6)
     zr = orc.prec_apply(_o(A), "ilu", _o(F), r)
     rt = torch.tensor(r, device=dev)

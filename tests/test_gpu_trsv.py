@@ -332,6 +332,49 @@ def test_tiled_schedule_wide_rows(dev, dims):
     assert np.linalg.norm(z32.double().cpu().numpy() - ref) <= 1e-4 * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("dims", [(24, 20, 16), (17, 13, 11), (40, 70, 6)])
+def test_tiled_schedule_ilu1_27pt(dev, dims):
+    """ILU(1) of a natural-order 27-point grid: rows of up to 31 entries run
+    the tiled kernel's four-lane form on tiles of up to 16 lines of one plane
+    (the box grid recognised from the row indices; several tiles per plane
+    at 40 x 70 x 6). Both triangles match the oracle in FP64 (1e-12), the
+    level launches (1e-13) and repeat bitwise; the FP32 apply stays within
+    1e-5 of the FP64 one (its storage rounding gives about 2e-7 here)."""
+    import torch
+    A = _o(la.gen.aniso27(*dims))
+    F = orc.iluk(A, 1)[0]
+    P = la.Preconditioner(as_lib_csr(F), "ilu")
+    tl, tu = P.tiles()
+    print("case aniso27 ilu1", dims, "tiles", tl, tu, flush=True)
+    assert tl > 1 and tu > 1
+    b = la.gen.uniform(F.n, 16)
+    ref = orc.prec_apply(F, "ilu", F, b)
+    bt = torch.tensor(b, device=dev)
+    z = torch.zeros_like(bt)
+    P.apply(bt, z, mode=la.TRSV_SYNCFREE)
+    assert _close(z.cpu().numpy(), ref, 1e-12)
+    zl = torch.zeros_like(bt)
+    P.apply(bt, zl, mode=la.TRSV_LEVEL)
+    assert _close(z.cpu().numpy(), zl.cpu().numpy(), 1e-13)
+    for _ in range(3):
+        z2 = torch.zeros_like(bt)
+        P.apply(bt, z2, mode=la.TRSV_SYNCFREE)
+        assert torch.equal(z2, z)
+    P.add_dtype(la.F32)
+    z32 = torch.zeros(F.n, dtype=torch.float32, device=dev)
+    P.apply(bt.float(), z32, mode=la.TRSV_SYNCFREE, val=la.F32)
+    assert np.linalg.norm(z32.double().cpu().numpy() - ref) <= 1e-5 * np.linalg.norm(ref)
+    L, D, U = orc.split_ldu(F)
+    fw = la.TriangularSolve(as_lib_csr(L), None, upper=False)
+    bw = la.TriangularSolve(as_lib_csr(U), D, upper=True)
+    assert fw.info()["tiled"] and bw.info()["tiled"]
+    x = torch.zeros_like(bt)
+    fw.solve(bt, x, mode=la.TRSV_SYNCFREE)
+    assert _close(x.cpu().numpy(), orc.forward_sub(L, b), 1e-12)
+    bw.solve(bt, x, mode=la.TRSV_SYNCFREE)
+    assert _close(x.cpu().numpy(), orc.backward_sub(U, D, b), 1e-12)
+
+
 def test_tiled_schedule_off_and_non_grid(dev, monkeypatch):
     """LAAMD_SF_TILED=0 at plan creation keeps the level-chunk schedule;
     a pattern that is not a grid keeps it too; both solve exactly."""
