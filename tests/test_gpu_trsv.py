@@ -332,20 +332,30 @@ def test_tiled_schedule_wide_rows(dev, dims):
     assert np.linalg.norm(z32.double().cpu().numpy() - ref) <= 1e-4 * np.linalg.norm(ref)
 
 
-@pytest.mark.parametrize("dims", [(24, 20, 16), (17, 13, 11), (40, 70, 6)])
-def test_tiled_schedule_ilu1_27pt(dev, dims):
-    """ILU(1) of a natural-order 27-point grid: rows of up to 31 entries run
-    the tiled kernel's four-lane form on tiles of up to 16 lines of one plane
-    (the box grid recognised from the row indices; several tiles per plane
-    at 40 x 70 x 6). Both triangles match the oracle in FP64 (1e-12), the
-    level launches (1e-13) and repeat bitwise; the FP32 apply stays within
-    1e-5 of the FP64 one (its storage rounding gives about 2e-7 here)."""
+def _box_cases():
+    """Natural-order grids whose ILU factors have rows of 9-32 entries:
+    27-point ILU(1) (several tiles per plane at 40 x 70 x 6), 27-point
+    ILU(0) and 7-point ILU(2)."""
+    return [("aniso27_24x20x16_ilu1", lambda: orc.iluk(_o(la.gen.aniso27(24, 20, 16)), 1)[0]),
+            ("aniso27_17x13x11_ilu1", lambda: orc.iluk(_o(la.gen.aniso27(17, 13, 11)), 1)[0]),
+            ("aniso27_40x70x6_ilu1", lambda: orc.iluk(_o(la.gen.aniso27(40, 70, 6)), 1)[0]),
+            ("aniso27_18x14x10_ilu0", lambda: orc.iluk(_o(la.gen.aniso27(18, 14, 10)), 0)[0]),
+            ("laplace3d_20x16x12_ilu2", lambda: orc.iluk(_o(la.gen.laplace3d(20, 16, 12)), 2)[0])]
+
+
+@pytest.mark.parametrize("name,make", _box_cases(), ids=[c[0] for c in _box_cases()])
+def test_tiled_schedule_box_grid(dev, name, make):
+    """ILU factors of natural-order grids with rows of up to 32 entries run
+    the tiled kernel's four-lane form on tiles of 8 x 2 lines skewed in
+    (j + 2k, k), the box grid recognised from the row indices. Both
+    triangles match the oracle in FP64 (1e-12), the level launches (1e-13)
+    and repeat bitwise; the FP32 apply stays within 1e-5 of the FP64 one
+    (its storage rounding gives about 2e-7 here)."""
     import torch
-    A = _o(la.gen.aniso27(*dims))
-    F = orc.iluk(A, 1)[0]
+    F = make()
     P = la.Preconditioner(as_lib_csr(F), "ilu")
     tl, tu = P.tiles()
-    print("case aniso27 ilu1", dims, "tiles", tl, tu, flush=True)
+    print("case", name, "tiles", tl, tu, flush=True)
     assert tl > 1 and tu > 1
     b = la.gen.uniform(F.n, 16)
     ref = orc.prec_apply(F, "ilu", F, b)
