@@ -249,7 +249,8 @@ int la_trsv_add_dtype(la_trsv_t T, la_dtype dt);
 int la_trsv_set_stream(la_trsv_t T, void *hip_stream);
 int la_trsv_info(la_trsv_t T, int *n, int *nlevels, int64_t *nnz);
 /* Which schedule the sync-free mode runs (no counterpart in the reference):
- * *tiled = 1 when the triangle's pattern was recognised as a grid and cut
+ * *tiled = 1 when the triangle's pattern was recognised as a grid (from the
+ * pattern, or for a natural-order box grid from the row indices) and cut
  * into *tiles tiles of edge *tile_edge grid steps, one workgroup each (the
  * environment variable LAAMD_SF_TILED=0 at plan creation turns it off);
  * 0 for the level-chunk schedule. */
