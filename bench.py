@@ -57,7 +57,7 @@ ALG_DESC = {0: "one wavefront per row", 2: "nnz-balanced segment kernel, 512 nnz
             "wavefront, + boundary-row fix-up",
             4: "sliced ELL, 64-row column-major slices, one lane per row, 16-bit column "
                "offsets where they fit"}
-ALG_KERNEL = {0: "k_spmv_row<G,double,double,double>",
+ALG_KERNEL = {0: "k_spmv_row<G,RB,double,double,double>",
               2: "k_spmv_seg<double,double,double> + k_spmv_seg_fix (one SpMV; duration "
                  "of the pair)",
               4: "k_spmv_sell<%s,double,double,double>"}

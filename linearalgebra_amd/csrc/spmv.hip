@@ -36,24 +36,60 @@ __device__ __forceinline__ V ldx(const V *x, int j) {
 }
 
 // ------------------------------------------------------------------ row
-template <int G, typename T, typename V, typename A>
+// G lanes per row (lane sub adds entries sub, sub + G, ... in order, then a
+// DPP sum over the G lanes); each lane group takes RB rows, RB * 64 / G
+// consecutive rows per wavefront, and issues the first entry's loads of all
+// of them before any product: with rows of a few entries a wavefront
+// otherwise has too few bytes in flight to keep HBM busy.
+template <int G, int RB, typename T, typename V, typename A>
 __global__ __launch_bounds__(kBlock) void k_spmv_row(int n, const int *__restrict__ ai,
                                                      const int *__restrict__ aj,
                                                      const T *__restrict__ av,
                                                      const V *__restrict__ x, V *y,
                                                      const V *z, A alpha, A beta) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const int64_t row = t / G;
-  const int sub = threadIdx.x & (G - 1);
-  if (row >= n) return;  // a whole group leaves together (G divides 64)
-  const int rs = ai[row], re = ai[row + 1];
-  A acc = 0;
-  for (int q = rs + sub; q < re; q += G) acc += A(ldm(av, q)) * A(ldx(x, ldi(aj, q)));
-  acc = group_sum<G>(acc);
-  if (sub == 0) {
-    A r = alpha * acc;
-    if (beta != A(0)) r += beta * A(z[row]);
-    y[row] = V(r);
+  constexpr int NG = 64 / G;  // lane groups per wavefront
+  const int lane = threadIdx.x & 63, g = lane / G, sub = lane & (G - 1);
+  const int64_t base =
+      (static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6)) * NG * RB + g;
+  int rs[RB], re[RB], c[RB];
+  A v[RB];
+#pragma unroll
+  for (int k = 0; k < RB; ++k) {
+    const int64_t row = base + k * NG;
+    rs[k] = re[k] = 0;
+    if (row < n) {
+      rs[k] = ai[row];
+      re[k] = ai[row + 1];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < RB; ++k) {
+    const int q = rs[k] + sub;
+    c[k] = 0;
+    v[k] = A(0);
+    if (q < re[k]) {
+      c[k] = ldi(aj, q);
+      v[k] = A(ldm(av, q));
+    }
+  }
+  A acc[RB];
+#pragma unroll
+  for (int k = 0; k < RB; ++k) {
+    acc[k] = A(0);
+    if (rs[k] + sub < re[k]) acc[k] += v[k] * A(ldx(x, c[k]));
+  }
+#pragma unroll
+  for (int k = 0; k < RB; ++k)
+    for (int q = rs[k] + sub + G; q < re[k]; q += G) acc[k] += A(ldm(av, q)) * A(ldx(x, ldi(aj, q)));
+#pragma unroll
+  for (int k = 0; k < RB; ++k) {
+    const int64_t row = base + k * NG;
+    acc[k] = group_sum<G>(acc[k]);
+    if (sub == 0 && row < n) {
+      A r = alpha * acc[k];
+      if (beta != A(0)) r += beta * A(z[row]);
+      y[row] = V(r);
+    }
   }
 }
 
@@ -384,12 +420,19 @@ void launch_row(Mat &M, const T *av, const V *x, V *y, const V *z, A alpha, A be
   const int n = M.rows;
   int G = 1;
   while (G < 64 && G < M.avg_row) G <<= 1;
-  const int64_t threads = static_cast<int64_t>(n) * G;
-  const unsigned grid = static_cast<unsigned>((threads + kBlock - 1) / kBlock);
-  if (grid == 0) return;
   const int *ai = M.ai.as<int>(), *aj = M.aj.as<int>();
-#define LA_ROW(GG) \
-  hipLaunchKernelGGL((k_spmv_row<GG, T, V, A>), dim3(grid), dim3(kBlock), 0, s, n, ai, aj, av, x, y, z, alpha, beta)
+  // rows per lane group, measured (FP64): 128^3 7-point (G 8) 72.8 us with
+  // 1, 52.2 with 4, 66.1 with 8; 160^3 27-point (G 32) 538 us with 1, 453
+  // with 2; the 4M power-law matrix (G 16) 1099 us with 1, 1116 with 2
+#define LA_ROW(GG)                                                                           \
+  {                                                                                          \
+    constexpr int RB = GG <= 8 ? 4 : GG == 32 ? 2 : 1;                                       \
+    const int64_t per = static_cast<int64_t>(kBlock / GG) * RB;                              \
+    const unsigned grid = static_cast<unsigned>((n + per - 1) / per);                        \
+    if (grid)                                                                                \
+      hipLaunchKernelGGL((k_spmv_row<GG, RB, T, V, A>), dim3(grid), dim3(kBlock), 0, s, n, ai, \
+                         aj, av, x, y, z, alpha, beta);                                      \
+  }
   switch (G) {
     case 1: LA_ROW(1); break;
     case 2: LA_ROW(2); break;
