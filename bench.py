@@ -19,7 +19,8 @@ int32 indices, RCM-reordered on the host.
     solves and with 3 Jacobi sweeps per triangle.
   * roofline: the SpMV kernel against the 8 TB/s HBM peak, its average
     launch duration from one pair of HIP events around the K timed launches
-    on the stream they run on (so gaps between launches count too).
+    on the stream they run on (so gaps between launches count too); beside
+    it a measured ceiling, a 1 GiB device-to-device copy.
   * cpu_baseline: the oracle's OpenMP SpMV (a restatement of the reference's
     ParallelSPMV) on this host's cores, the median of single SpMVs timed
     over a bounded ~10 s sample after one warm-up, with the CPU model and
@@ -120,6 +121,40 @@ def time_spmv(la, torch, mats, x, y, steps, warmup, alg, val, acc, stream):
     torch.cuda.synchronize()
     wall_ms = (time.perf_counter() - t0) * 1e3
     return [e0.elapsed_time(e1) / steps], wall_ms
+
+
+def copy_ceiling(torch, dev, mib=1024, reps=10):
+    """Measured bandwidth ceilings next to the 8 TB/s peak: a device-to-device
+    copy of `mib` MiB (that many bytes read and as many written) and a sum
+    over the same buffer (read only, as an SpMV nearly is), each timed one
+    call at a time with HIP events; the medians."""
+    src = torch.empty(mib << 17, dtype=torch.float64, device=dev)
+    src.fill_(1.0)
+    dst = torch.empty_like(src)
+
+    def med(fn):
+        fn()
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1) * 1e-3)
+        return float(np.median(times))
+
+    t = med(lambda: dst.copy_(src))
+    tr = med(lambda: torch.sum(src))
+    nbytes, rbytes = 2 * (mib << 20), mib << 20
+    del src, dst
+    return {"what": "%d MiB device-to-device copy (read + write), median of %d" % (mib, reps),
+            "bytes": nbytes, "us": round(t * 1e6, 2), "GBs": round(nbytes / t / 1e9, 1),
+            "frac_of_peak": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4),
+            "read": {"what": "torch.sum over the same %d MiB (read only)" % mib,
+                     "us": round(tr * 1e6, 2), "GBs": round(rbytes / tr / 1e9, 1),
+                     "frac_of_peak": round(rbytes / tr / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 def cpu_info():
@@ -723,6 +758,10 @@ def main():
                                                       else "int32"),
                   "bytes_per_spmv": int(Bl), "GBs": round(Bl / (avg_ms * 1e-3) / 1e9, 1),
                   "frac": round(Bl / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    copy = copy_ceiling(torch, dev)
+    if layout:
+        layout["frac_of_copy"] = round(layout["GBs"] / copy["GBs"], 4)
+        layout["frac_of_read"] = round(layout["GBs"] / copy["read"]["GBs"], 4)
     traffic = None
     tpath = os.path.join(ROOT, "profiles",
                          "traffic_spmv_%s_f64_laplace3d_128.json" % ALG_NAMES[chosen])
@@ -826,7 +865,8 @@ def main():
                      "traffic_GBs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
                      "traffic_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
                      if traffic else None,
-                     "layout": layout},
+                     "layout": layout,
+                     "copy_ceiling": copy},
         "cpu_baseline": cpu,
         "pcg": pcg,
     }
