@@ -206,8 +206,8 @@ __global__ __launch_bounds__(NT) void k_trsv_syncfree(
 // id) kTileDepth groups ahead of the group it solves, the row ids twice as
 // far ahead. A group's dependency chain is then the x reads from LDS, the
 // products, and the LDS write and HBM store of the results (padding lanes
-// store nothing). Wavefronts 1..kTileHalo fetch the other tiles' values:
-// wavefront w takes the tile's groups w-1, w-1+kTileHalo, ..., polls the
+// store nothing). Wavefronts 1..H fetch the other tiles' values:
+// wavefront w takes the tile's groups w-1, w-1+H, ..., polls the
 // values each group reads first (agent-scope loads of x, as in the
 // level-chunk kernel) into the halo, loading the next group's list while
 // it polls, and counts its finished groups in `s_prog[w-1]`; the solver
@@ -218,8 +218,16 @@ __global__ __launch_bounds__(NT) void k_trsv_syncfree(
 // the order in the code); the fetching wavefronts wait for their LDS writes
 // before counting a group. Every wait is bounded like the level-chunk
 // kernel's; a timeout sets *err.
+//
+// H, the number of fetching wavefronts, trades halo throughput against the
+// solving wavefront's share of its SIMD: with 8 wavefronts every SIMD holds
+// two, and the solver shares issue with a fetcher; with 4, the solver has
+// its SIMD to itself. Rows on one lane (2-D and 7-point tiles, many halo
+// values per group) take 7; rows on kTileRowLanes lanes (box tiles, 16 rows
+// per group) take 3.
 constexpr int kTileLanes = 64;
-constexpr int kTileHalo = 7;    // halo wavefronts per workgroup (8 in all)
+constexpr int kTileHalo = 7;      // fetching wavefronts for rows on one lane
+constexpr int kTileHaloWide = 3;  // for rows on kTileRowLanes lanes
 constexpr int kTileDepth = 8;   // groups whose rows sit in registers
 constexpr int kTileXl = kTileWindow * kTileLanes;  // result slots; kTileXl is the zero slot
 
@@ -269,15 +277,14 @@ struct TileStage {
   T dinv;
 };
 
-template <int E, int RL, typename T, typename V, typename A, bool DIAG>
-__global__ __launch_bounds__(kTileLanes * (1 + kTileHalo)) void k_trsv_tiled(
+template <int E, int RL, int H, typename T, typename V, typename A, bool DIAG>
+__global__ __launch_bounds__(kTileLanes * (1 + H)) void k_trsv_tiled(
     int nparts, const int *__restrict__ gptr, const int *__restrict__ prow,
     const uint16_t *__restrict__ pcode, const T *__restrict__ pval, const T *__restrict__ pdinv,
     const int *__restrict__ rfptr, const int *__restrict__ rfcol, const int *__restrict__ rfslot,
     const V *__restrict__ b, V *x, unsigned *ctl, int *err) {
   // pipeline depth: slots of 8 entries hold twice the registers
   constexpr int D = E <= 4 ? kTileDepth : kTileDepth * 3 / 4;
-  constexpr int H = kTileHalo;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_prog[H];
   __shared__ int s_part, s_late;
@@ -496,9 +503,9 @@ inline size_t tile_lds_bytes(int h_max, size_t vsize) {
   return (static_cast<size_t>(kTileXl) + 1 + h_max) * vsize;
 }
 
-template <int E, int RL, typename T, typename V, typename A, bool DIAG>
+template <int E, int RL, int H, typename T, typename V, typename A, bool DIAG>
 void tiled_launch(TrsvPlan &P, const V *b, V *x, hipStream_t s) {
-  auto kern = k_trsv_tiled<E, RL, T, V, A, DIAG>;
+  auto kern = k_trsv_tiled<E, RL, H, T, V, A, DIAG>;
   const size_t lds = tile_lds_bytes(P.t_hmax, sizeof(V));
   static size_t lds_set = 0;  // per instantiation: the largest size allowed so far
   if (lds > lds_set) {
@@ -512,7 +519,7 @@ void tiled_launch(TrsvPlan &P, const V *b, V *x, hipStream_t s) {
   const unsigned fg = static_cast<unsigned>(std::min<int64_t>(4096, (P.n + kB - 1) / kB));
   hipLaunchKernelGGL(k_tile_prep<V>, dim3(std::max(fg, 1u)), dim3(kB), 0, s,
                      static_cast<int64_t>(P.n), x, P.counter.as<unsigned>());
-  hipLaunchKernelGGL(kern, dim3(P.t_wgs), dim3(kTileLanes * (1 + kTileHalo)), lds, s,
+  hipLaunchKernelGGL(kern, dim3(P.t_wgs), dim3(kTileLanes * (1 + H)), lds, s,
                      P.t_nparts, P.t_gptr.as<int>(), P.t_prow.as<int>(),
                      P.t_pcode.as<uint16_t>(), tv, td, P.t_rfptr.as<int>(), P.t_rfcol.as<int>(),
                      P.t_rfslot.as<int>(), b, x, P.counter.as<unsigned>(), P.err_flag.as<int>());
@@ -522,9 +529,10 @@ template <typename T, typename V, typename A, bool DIAG>
 void launch_syncfree(TrsvPlan &P, const T *val, const T *diag, const V *b, V *x, hipStream_t s) {
   const int64_t n = P.n;
   if (P.tiled) {
-    if (P.t_maxe == 4) tiled_launch<4, 1, T, V, A, DIAG>(P, b, x, s);
-    else if (P.t_rlanes == 1) tiled_launch<kTileMaxEntries, 1, T, V, A, DIAG>(P, b, x, s);
-    else tiled_launch<kTileMaxEntries, kTileRowLanes, T, V, A, DIAG>(P, b, x, s);
+    constexpr int E8 = kTileMaxEntries, RW = kTileRowLanes;
+    if (P.t_maxe == 4) tiled_launch<4, 1, kTileHalo, T, V, A, DIAG>(P, b, x, s);
+    else if (P.t_rlanes == 1) tiled_launch<E8, 1, kTileHalo, T, V, A, DIAG>(P, b, x, s);
+    else tiled_launch<E8, RW, kTileHaloWide, T, V, A, DIAG>(P, b, x, s);
     LA_HIP(hipGetLastError());
     return;
   }
