@@ -558,20 +558,20 @@ RCM_PINS = {
     # width 2 and win, then 3 > 2 swaps the ends: [4,3,2,0,1]. With the
     # original degrees the children sort to 4, 1, 2: [2,1,4,0,3].
     "hub5": (5, [(0, 1), (0, 2), (0, 3), (0, 4), (1, 2)], [4, 2, 1, 0, 3]),
-    # Degrees 4,6,3,3,3,3,3,3. Source 2. Round 1: BFS(2): {2} {0,1}
+    # Degrees 4,6,3,3,3,3,3,3. Source 2. Search pass 1: BFS(2): {2} {0,1}
     # {5,3,4,7} {6}: height 4, width 4, last {6}. First five: 6 -> 6, 4, 7
     # overwritten; candidates [6]. BFS(6): {6} {4,7} {1} {0,2,3} {5}:
     # height 5 > 4 and width 4 < INT_MAX -> new source 6, no target yet.
-    # Round 2 (new BFS object): BFS(6) width 3, height 5, last {5}. First
+    # Search pass 2 (new BFS object): BFS(6) width 3, height 5, last {5}. First
     # five: 5 -> 5, 0, 3 overwritten; candidates [5]. BFS(5): {5} {0,3}
     # {1,2} {4,7} {6}: height 5 (not > 5), width 3 -> target 5; 3 > 3
     # false. CM from 6: {4,7} (both INT_MAX, by index), 4 -> {1}, 1 ->
     # {0,2,3} sorted by degree (2: 3; 0, 3: INT_MAX) -> 2, 0, 3; 0 -> {5}:
     # [6,4,7,1,2,0,3,5], reversed [5,3,0,2,1,7,4,6]. Original degrees give
     # 2, 3, 0 ([5,0,3,2,1,7,4,6]); index order 0, 2, 3 ([5,3,2,0,1,7,4,6]);
-    # a width reset per BFS makes round 2's backward width 2 < 3 and swaps
+    # a width reset per BFS makes pass 2's backward width 2 < 3 and swaps
     # the ends ([6,7,4,1,2,3,0,5]).
-    "two_rounds8": (8, [(0, 1), (0, 2), (0, 5), (1, 2), (1, 3), (1, 4), (1, 7), (3, 5),
+    "two_passes8": (8, [(0, 1), (0, 2), (0, 5), (1, 2), (1, 3), (1, 4), (1, 7), (3, 5),
                         (4, 6), (6, 7)], [5, 3, 0, 2, 1, 7, 4, 6]),
     # Four components; roots (largest member, UnionFind.cpp:82-129) 7, 8,
     # 10, 11 give the order {7}, {0,2,5,8}, {3,10}, {1,4,6,9,11}
@@ -600,8 +600,8 @@ def test_rcm_hand_derived_orders(name, base):
 # ----------------------------------------------------------------- planted
 
 @pytest.mark.parametrize("plant,caught", [
-    (orc.PLANT_RCM_ORIGINAL_DEGREES, {"cycle5", "hub5", "two_rounds8", "components12"}),
-    (orc.PLANT_RCM_WIDTH_RESET, {"hub5", "two_rounds8"}),
+    (orc.PLANT_RCM_ORIGINAL_DEGREES, {"cycle5", "hub5", "two_passes8", "components12"}),
+    (orc.PLANT_RCM_WIDTH_RESET, {"hub5", "two_passes8"}),
 ])
 def test_rcm_pins_catch_planted_mistakes(plant, caught):
     """A child sort by the unmodified degrees, or a BFS width reset per call,
