@@ -220,13 +220,14 @@ __global__ __launch_bounds__(NT) void k_trsv_syncfree(
 // kernel's; a timeout sets *err.
 //
 // H, the number of fetching wavefronts, trades halo throughput against the
-// solving wavefront's share of its SIMD: with 8 wavefronts every SIMD holds
-// two, and the solver shares issue with a fetcher; with 4, the solver has
-// its SIMD to itself. Rows on one lane (2-D and 7-point tiles, many halo
-// values per group) take 7; rows on kTileRowLanes lanes (box tiles, 16 rows
+// solving wavefront's share of its SIMD and the memory traffic of the
+// polls: with 7 or 8 wavefronts most SIMDs hold two, and the solver may
+// share issue with a fetcher; with 4, the solver has its SIMD to itself.
+// Rows on one lane (2-D and 7-point tiles, up to 64 rows and many halo
+// values per group) take 6; rows on kTileRowLanes lanes (box tiles, 16 rows
 // per group) take 3.
 constexpr int kTileLanes = 64;
-constexpr int kTileHalo = 7;      // fetching wavefronts for rows on one lane
+constexpr int kTileHalo = 6;      // fetching wavefronts for rows on one lane
 constexpr int kTileHaloWide = 3;  // for rows on kTileRowLanes lanes
 constexpr int kTileDepth = 8;   // groups whose rows sit in registers
 constexpr int kTileXl = kTileWindow * kTileLanes;  // result slots; kTileXl is the zero slot
