@@ -385,6 +385,60 @@ def test_tiled_schedule_box_grid(dev, name, make):
     assert _close(x.cpu().numpy(), orc.backward_sub(U, D, b), 1e-12)
 
 
+def test_tiled_schedule_box_grid_base1(dev):
+    """The box grid is recognised the same way from 1-based indices: both
+    triangles of a 1-based copy run tiled and give the 0-based solves'
+    results bit for bit."""
+    import torch
+    F = orc.iluk(_o(la.gen.aniso27(17, 13, 11)), 1)[0]
+    L, D, U = orc.split_ldu(F)
+    b = la.gen.uniform(F.n, 18)
+    bt = torch.tensor(b, device=dev)
+    for T, diag, upper in ((L, None, False), (U, D, True)):
+        A0 = as_lib_csr(T)
+        A1 = la.CSR(T.n, np.asarray(T.ai) + 1, np.asarray(T.aj) + 1, T.av, ncols=T.ncols)
+        s0 = la.TriangularSolve(A0, diag, upper=upper)
+        s1 = la.TriangularSolve(A1, diag, upper=upper)
+        print("case aniso27_17x13x11_ilu1 base1", upper, s1.info(), flush=True)
+        assert s0.info()["tiled"] and s1.info()["tiled"]
+        assert s1.info()["tiles"] == s0.info()["tiles"]
+        x0, x1 = torch.zeros_like(bt), torch.zeros_like(bt)
+        s0.solve(bt, x0, mode=la.TRSV_SYNCFREE)
+        s1.solve(bt, x1, mode=la.TRSV_SYNCFREE)
+        assert torch.equal(x0, x1)
+        ref = orc.backward_sub(U, D, b) if upper else orc.forward_sub(L, b)
+        assert _close(x1.cpu().numpy(), ref, 1e-12)
+
+
+def test_box_grid_rows_too_long_keep_level_chunks(dev):
+    """A natural-order box grid whose factor has rows longer than the tiled
+    kernel's four lanes take (27-point ILU(2): up to 57 entries) is refused
+    by the tiled layout; both triangles keep the level-chunk schedule and
+    solve exactly (oracle at 1e-12, the level launches at 1e-13, bitwise
+    repeats)."""
+    import torch
+    F = orc.iluk(_o(la.gen.aniso27(16, 14, 12)), 2)[0]
+    L, D, U = orc.split_ldu(F)
+    assert max(np.diff(np.asarray(L.ai)).max(), np.diff(np.asarray(U.ai)).max()) > 32
+    fw = la.TriangularSolve(as_lib_csr(L), None, upper=False)
+    bw = la.TriangularSolve(as_lib_csr(U), D, upper=True)
+    print("case aniso27_16x14x12_ilu2", fw.info(), bw.info(), flush=True)
+    assert not fw.info()["tiled"] and not bw.info()["tiled"]
+    b = la.gen.uniform(F.n, 17)
+    bt = torch.tensor(b, device=dev)
+    for T, ref in ((fw, orc.forward_sub(L, b)), (bw, orc.backward_sub(U, D, b))):
+        x = torch.zeros_like(bt)
+        T.solve(bt, x, mode=la.TRSV_SYNCFREE)
+        assert _close(x.cpu().numpy(), ref, 1e-12)
+        xl = torch.zeros_like(bt)
+        T.solve(bt, xl, mode=la.TRSV_LEVEL)
+        assert _close(x.cpu().numpy(), xl.cpu().numpy(), 1e-13)
+        for _ in range(3):
+            x2 = torch.zeros_like(bt)
+            T.solve(bt, x2, mode=la.TRSV_SYNCFREE)
+            assert torch.equal(x2, x)
+
+
 def test_tiled_schedule_off_and_non_grid(dev, monkeypatch):
     """LAAMD_SF_TILED=0 at plan creation keeps the level-chunk schedule;
     a pattern that is not a grid keeps it too; both solve exactly."""
