@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 import linearalgebra_amd as la
-from helpers import ROOT, orc
+from helpers import ROOT, orc, random_box_triangle
 
 LANES, WINDOW = 64, 16
 ZERO_SLOT = WINDOW * LANES
@@ -149,42 +149,6 @@ def test_tiled_layout_replays_to_substitution(tl, name, make, rl):
         assert np.abs(x - ref).max() <= 1e-12 * np.abs(ref).max(), name
 
 
-# the lower-row offsets (di, dj, dk) of a 27-point stencil's ILU(1) factor
-_ILU1_27 = [(di, dj, -1) for dj in range(-1, 3) for di in range(-2, 3)] + \
-    [(di, dj, 0) for dj in (-2, -1) for di in range(-2, 3)] + [(-2, 0, 0), (-1, 0, 0)]
-
-
-def _random_box_triangle(nx, ny, nz, seed, upper):
-    """A strict triangle on a natural-order box grid whose rows hold a
-    random subset of the 27-point ILU(1) lower-row offsets (always the
-    neighbours one step back along each axis), with random values."""
-    rng = np.random.default_rng(seed)
-    must = [(-1, 0, 0), (0, -1, 0), (0, 0, -1)]
-    rest = [o for o in _ILU1_27 if o not in must]
-    rows, cols = [], []
-    for k in range(nz):
-        for j in range(ny):
-            for i in range(nx):
-                r = i + nx * (j + ny * k)
-                cs = set()
-                for di, dj, dk in must + [o for o in rest if rng.random() < 0.6]:
-                    ii, jj, kk = i + di, j + dj, k + dk
-                    if 0 <= ii < nx and 0 <= jj < ny and 0 <= kk < nz:
-                        cs.add(ii + nx * (jj + ny * kk))
-                for c in sorted(cs):
-                    rows.append(r)
-                    cols.append(c)
-    n = nx * ny * nz
-    rows, cols = np.array(rows), np.array(cols)
-    if upper:  # mirror the pattern: row n-1-r reads n-1-c
-        rows, cols = n - 1 - rows, n - 1 - cols
-        o = np.lexsort((cols, rows))
-        rows, cols = rows[o], cols[o]
-    ai = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]).astype(np.int32)
-    av = rng.uniform(-1, 1, len(cols)) / 64  # rows of at most 31: diagonally dominant
-    return orc.CSR(n, ai, cols.astype(np.int32), av)
-
-
 @pytest.mark.parametrize("seed", [1, 2])
 @pytest.mark.parametrize("upper", [False, True])
 def test_tiled_layout_random_box_patterns(tl, seed, upper):
@@ -193,7 +157,7 @@ def test_tiled_layout_random_box_patterns(tl, seed, upper):
     exactly, whichever entries each row holds."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spl
-    T = _random_box_triangle(13, 11, 9, seed, upper)
+    T = random_box_triangle(13, 11, 9, seed, upper)
     D = 1.0 + np.random.default_rng(seed).random(T.n)
     b = la.gen.uniform(T.n, 22)
     x, nparts, rl = _replay(tl, T, D, b, upper)
