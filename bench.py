@@ -326,9 +326,15 @@ def run_pcg(la, torch, A, dev, stream, trsv_mode, variants, cpu=False):
 
 
 def _spmv_table(la, torch, A, mats, dev, stream, algs, dtypes, steps, seed=77):
-    """GB/s and relative error against the FP64 oracle for each (alg, dtype)."""
+    """GB/s and relative error against the FP64 oracle for each (alg, dtype).
+    GBs counts the CSR algorithmic bytes for that value type; where AUTO
+    runs the sliced-ELL copy, layout_GBs / layout_frac count the bytes that
+    copy streams itself (every slot's value and column offset, the slice
+    table, x once and y)."""
     from oracle import oracle as orc
     n, nnz = A.n, A.nnz
+    sinfo = mats[0].sell_info()
+    sell = mats[0].resolve(la.SPMV_AUTO) == 4 and sinfo["built"]
     x64 = torch.tensor(la.gen.uniform(n, seed), device=dev)
     ref = orc.spmv(orc.CSR(n, A.ai, A.aj, A.av), x64.cpu().numpy(), alg="parallel",
                    nthreads=orc.max_threads())
@@ -350,8 +356,14 @@ def _spmv_table(la, torch, A, mats, dev, stream, algs, dtypes, steps, seed=77):
             sx = 8 if dt == la.F64 else 4
             B = spmv_bytes(n, nnz, s_v=sv, s_x=sx, s_y=sx)
             us = float(np.mean(d)) * 1e3
-            out["%s_%s" % (aname, name)] = {"us": round(us, 2), "GBs": round(B / us / 1e3, 1),
-                                            "rel_err_vs_fp64": float("%.3e" % err)}
+            rec = {"us": round(us, 2), "GBs": round(B / us / 1e3, 1),
+                   "rel_err_vs_fp64": float("%.3e" % err)}
+            if alg == la.SPMV_AUTO and sell:
+                Bl = (sinfo["slots"] * (sv + (2 if sinfo["col16"] else 4)) + (n + 63) // 64 * 12
+                      + 2 * n * sx)
+                rec["layout_GBs"] = round(Bl / us / 1e3, 1)
+                rec["layout_frac"] = round(Bl / us / 1e3 / HBM_PEAK_GBS, 4)
+            out["%s_%s" % (aname, name)] = rec
             log(aname, name, out["%s_%s" % (aname, name)])
     return out
 
